@@ -1,0 +1,64 @@
+// dag.hpp -- gate dependency DAG with peephole merges and greedy
+// target-bounded block selection.
+//
+// Behaviour restated from the reference's src/cpp/DAG.cpp:
+//   add()     per-qubit barrier stacks; a new gate merges into the latest gate
+//             on its qubits when both masks are equal and that gate has no
+//             successor yet (:22-217): phase types add modulo 8 and vanish at
+//             0; phase shifts add angles; 2x2 unitaries multiply; equal X, Y,
+//             Y*, H cancel; equal rotations add half angles or cancel.
+//   select()  depth-first walk from the front nodes taking every gate whose
+//             non-diagonal targets keep popcount(targetMask) <= limit; the
+//             taken set is closed under predecessors and is emitted in
+//             insertion order, capped at max_gates; everything else goes to a
+//             new DAG (:219-259).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "gates.hpp"
+
+namespace dgq {
+
+class Dag {
+public:
+    explicit Dag(int num_qubits, bool merge = true);
+
+    void add(const Gate &g);
+    bool empty() const { return alive_count_ == 0; }
+    size_t size() const { return alive_count_; }
+
+    // Move the selected gates to `out` (appended), grow `target_mask`; the
+    // unselected gates are returned as a new DAG (same merge setting).
+    Dag select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates);
+
+    // All surviving gates in insertion order (a valid topological order).
+    std::vector<Gate> gates() const;
+
+private:
+    struct Node {
+        Gate g;
+        std::vector<int> prev, rear;
+        uint64_t barrier = 0;
+        uint64_t tbits = 0;
+        size_t remain = 0;
+        bool visited = false, selected = false, alive = true;
+    };
+    bool try_merge(int top, Node &nn);
+    void erase_node(int top);
+    void dfs(int start, uint64_t &mask, int limit);
+
+    int nq_;
+    bool merge_;
+    size_t alive_count_ = 0;
+    std::vector<Node> nodes_;
+    std::vector<std::vector<int>> stacks_;
+    std::vector<int> front_;
+};
+
+// 2x2 matrix of a non-diagonal gate (rows r0c0, r0c1, r1c0, r1c1 as re,im),
+// as src/cpp/QuEST_backend.h:480-530 defines it.
+void gate_matrix(const Gate &g, double m[8]);
+
+}  // namespace dgq

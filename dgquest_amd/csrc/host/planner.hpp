@@ -1,0 +1,63 @@
+// planner.hpp -- turns a recorded gate list into fused sweeps for the engine.
+//
+// Replaces the reference's planning chain for the one-GPU case:
+//   splitSectionLocalViewIntoFusedGates  src/cpp/QuEST_backend.cpp:118-141
+//     (blocks: targetMask starts with the coalesced qubits and may grow to a
+//      tile; here the coalesced set is qubits 0..5 and the tile is 6 + k)
+//   launchKernel per-target regrouping   src/cpp/QuEST_gpu_kernel.cu:529-546
+//     (here: phases, each at most R register targets)
+//   localizeTargetBits / partitionFusedOperator / statevec_fused
+//                                        src/cpp/QuEST_gpu_kernel.cu:361-527
+//     (here: control and target qubits translated to register, lane, wave or
+//      tile-index bits; sin/cos on the host)
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dgq_engine.h"
+#include "gates.hpp"
+
+namespace dgq {
+
+// Shape of a fused sweep.  k: maximum number of target qubits above the six
+// coalesced ones per block (the fusion width); R/W: register and wave bits
+// of the kernel (tile = 6 + R + W qubits, k <= R + W); max_ops caps the gates
+// per block so long blocks stay memory-bound.
+struct TuneParams {
+    int k = 6;
+    int R = 4;
+    int W = 2;
+    int max_ops = 4096;
+};
+
+struct PlanStats {
+    int sweeps = 0;      // fused blocks (one full read + write of the state each)
+    int phases = 0;      // register phases over all blocks
+    int exchanges = 0;   // LDS re-distributions over all blocks
+    int ops = 0;         // device ops
+    int gates_in = 0;    // recorded gates before merging
+};
+
+struct Plan {
+    std::vector<dq_block> blocks;
+    std::vector<dq_phase> phases;
+    std::vector<dq_op> ops;
+    PlanStats st;
+};
+
+// Plan `gates` (physical qubit indices) on an n-qubit state.  The first block
+// generates the state instead of reading it when init_kind is
+// DQ_INIT_CLASSICAL / DQ_INIT_PLUS.
+void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind, uint64_t init_index,
+                  double init_value, const TuneParams &tp, Plan &out);
+
+// A pass that only moves qubits: afterwards physical qubit dst[i] holds what
+// physical qubit src[i] held.  All moved qubits must fit one tile.
+bool plan_permutation(int n, int prec, const std::vector<int> &src, const std::vector<int> &dst,
+                      const TuneParams &tp, Plan &out, std::string &err);
+
+// Clamp a tuning choice to an n-qubit state (tile = 6 + R + W <= n).
+TuneParams fit_shape(const TuneParams &tp, int n);
+
+}  // namespace dgq

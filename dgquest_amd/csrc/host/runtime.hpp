@@ -1,0 +1,55 @@
+// runtime.hpp -- register state and executor behind the QuEST C API.
+//
+// One GPU, one process.  The state of a register lives in HBM for its whole
+// life; gates are recorded (in physical qubit positions) and flushed as fused
+// sweeps when a result is read.  Replaces the reference's QuregState /
+// QuESTEnvValueDistributed pair (src/cpp/QuEST_backend.h:314-431) for the
+// resident-state case; MPI, task partitions and the page pool are gone.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dgq_engine.h"
+#include "gates.hpp"
+#include "planner.hpp"
+
+struct QuESTEnvValue {
+    int device = 0;
+    dgq::TuneParams tune;       // default fusion shape for new registers
+    bool tune_from_table = false;
+};
+
+struct QuregState {
+    int n = 0;
+    int prec = DQ_F64;
+    void *d_state = nullptr;
+    void *d_ws = nullptr;
+    size_t ws_cap = 0;
+    void *stream = nullptr;
+    std::vector<dgq::Gate> pending;   // physical qubit positions
+    int init_kind = DQ_INIT_LOAD;     // pending initialisation (fused into the next sweep)
+    uint64_t init_index = 0;
+    double init_value = 0.0;
+    std::vector<int> l2p, p2l;        // logical <-> physical qubit
+    bool probs_dirty = true;
+    std::vector<double> p0;           // per physical qubit, total at [n]
+    dgq::TuneParams tune;
+    dgq::PlanStats last_stats;
+    double last_flush_ms = 0.0;
+};
+
+namespace dgq {
+
+[[noreturn]] void engine_fatal(const char *what, int rc);
+void ensure_ws(QuregState *s, size_t bytes);
+void flush(QuregState *s);
+void ensure_probs(QuregState *s);
+void materialize_mapping(QuregState *s);
+
+// MT19937 (Matsumoto & Nishimura's mt19937ar), process-global like the
+// reference's src/mt19937ar.c:57-58.
+void mt_init_by_array(const unsigned long *key, int len);
+double mt_genrand_real1();
+
+}  // namespace dgq

@@ -1,0 +1,56 @@
+/* dgquest_ext.h -- engine controls beside the QuEST API.
+ *
+ * None of these exist in the reference; they expose what its
+ * time_predictor / QuEST.conf pair decided implicitly
+ * (src/cpp/time_predictor.cpp:232-395, src/cpp/QuEST_distributed.cpp:42-56):
+ * the fused-block shape, the plan statistics of the last flush, explicit
+ * flush / sync points for timing, and a planning-only call that needs no GPU.
+ * All return 0 on success and a negative value on a bad argument.
+ */
+#ifndef DGQUEST_EXT_H
+#define DGQUEST_EXT_H
+
+#include "QuEST.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fused-block shape for this register: k = max target qubits above the six
+ * coalesced ones per sweep, R/W = register / wave bits of the tile kernel
+ * (tile = 6 + R + W qubits), max_ops = gate cap per sweep.  Values are
+ * clamped to what the register size and the built kernels allow. */
+int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops);
+
+/* stats[0..7] = sweeps, phases, LDS exchanges, device ops, recorded gates,
+ * k, R, W of the last flush; *flush_ms = host time of that flush (planning
+ * and launch, not waiting for the GPU). */
+int dgq_last_plan_stats(Qureg qureg, int stats[8], double *flush_ms);
+
+/* Run the recorded gates now (enqueue; returns without waiting). */
+int dgq_flush(Qureg qureg);
+
+/* Wait until all work of this register has finished on the GPU. */
+int dgq_sync(Qureg qureg);
+
+/* Flush, then copy amplitudes start..start+count-1 (logical order) into
+ * out[2*count] as {re, im} doubles. */
+int dgq_read_state(Qureg qureg, unsigned long long start, unsigned long long count, double *out);
+
+/* Planning only, no GPU needed: plan `ngates` gate records on an n-qubit
+ * state with the given shape and report stats[0..7] as above. */
+typedef struct dgq_gate_rec {
+    int type;                /* numbering of src/cpp/QuEST_backend.h:89-114 */
+    int pad;
+    unsigned long long tmask;
+    unsigned long long cmask;
+    double op[8];
+} dgq_gate_rec;
+int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k, int R, int W, int max_ops,
+                  int stats[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGQUEST_EXT_H */
