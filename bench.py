@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain fused-sweep method on one MI355X.
+
+Workload: the 30-qubit layered random circuit in complex FP64 (16 GiB state),
+depth 24 -- per layer U3(theta, phi, lambda) on every qubit with angles
+uniform on [0, 2pi), then CZ on a seeded perfect matching -- seed 2, state
+starting at |0...0>.  One step = one whole execution of that circuit through
+the QuEST API: initZeroState, the 1080 recorded gates, flush (fused sweeps)
+on the GPU.  The state stays resident in HBM across steps.
+
+value = algorithmic HBM bytes of all timed steps / timed wall time, where a
+sweep moves 2 x 16 x 2^30 bytes (one read and one write of every amplitude)
+and the first sweep of a step, which generates |0...0> instead of reading,
+moves half of that.
+
+roofline: the sweep kernel's algorithmic bytes per launch over its average
+launch duration, timed with HIP events on the register's own stream.
+cpu_baseline: the CPU oracle (oracle/oracle.c, OpenMP, gate by gate) on a
+bounded sample of the same circuit on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "30q FP64 HBM GB/s (frac of peak); served circuits/s + p99 at fixed fidelity"
+HBM_PEAK_GBS = 8000.0
+
+# Fusion shapes tried by the autotuner: (k, R, W); k = targets above the six
+# coalesced qubits per sweep, tile = 6 + R + W qubits.
+CANDIDATES = [(3, 3, 0), (4, 4, 0), (5, 4, 1), (6, 4, 2), (6, 3, 3), (7, 4, 3)]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=24)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--shape", type=str, default="", help="k,R,W to skip the autotuner")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per sweep launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (profiles/sweep_traffic_30q.txt, line 'traffic_bytes_per_launch: X')."""
+    p = os.path.join(ROOT, "profiles", "sweep_traffic_30q.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(circ, n, seconds):
+    """Oracle gate by gate on a 2^n FP64 state until `seconds` have passed."""
+    import numpy as np
+
+    from oracle.binding import Oracle, build
+
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        build(ref=False)
+    orc = Oracle()
+    state = np.zeros(1 << n, dtype=np.complex128)
+    state[0] = 1.0
+    done = 0
+    t0 = time.perf_counter()
+    for g in circ:
+        orc.gate(state, g[0], g[1], g[2])
+        done += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    gate_bytes = 2 * 16 * (1 << n)  # one read + one write of the state per gate
+    return {
+        "value": round(done * gate_bytes / dt / 1e9, 2),
+        "unit": "GB/s",
+        "cores": orc.threads(),
+        "kind": "port",
+        "sample": f"first {done} of {len(circ)} gates of the same {n}-qubit circuit, FP64, gate by gate "
+                  f"(one state read+write per gate) in {dt:.1f} s; whole circuit extrapolates to "
+                  f"{dt / done * len(circ):.0f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the state-vector engine runs on a single MI355X).", file=sys.stderr)
+        sys.exit(2)
+
+    from dgquest_amd import circuits
+    from dgquest_amd import quest as Q
+    from dgquest_amd.engine import Engine
+
+    q64 = Q.load(2)
+    eng = Engine()
+    env = q64.createQuESTEnv()
+    n = args.qubits
+    circ = circuits.layered_u3_cz(n, args.depth, args.seed)
+    q = q64.createQureg(n, env)
+    stream = q64.dgq_stream(q)
+    e0, e1 = eng.event(), eng.event()
+
+    def one_step():
+        q64.initZeroState(q)
+        q64.apply_circuit(q, circ)
+        q64.dgq_flush(q)
+
+    def gpu_time_of_step():
+        q64.initZeroState(q)
+        q64.apply_circuit(q, circ)
+        eng.record(e0, stream)
+        q64.dgq_flush(q)
+        eng.record(e1, stream)
+        return eng.elapsed_ms(e0, e1)
+
+    # ---- autotuner: fusion shape with the least GPU time for this circuit
+    tuning = {}
+    if args.shape:
+        k, R, W = (int(x) for x in args.shape.split(","))
+        best = (k, R, W)
+        q64.dgq_set_tuning(q, k, R, W, 100000)
+    else:
+        for (k, R, W) in CANDIDATES:
+            q64.dgq_set_tuning(q, k, R, W, 100000)
+            gpu_time_of_step()  # first run of a shape warms it
+            ms = gpu_time_of_step()
+            st = q64.plan_stats(q)
+            tuning[f"k{k}_R{R}_W{W}"] = {"ms": round(ms, 2), "sweeps": st["sweeps"]}
+        best = min(CANDIDATES, key=lambda c: tuning[f"k{c[0]}_R{c[1]}_W{c[2]}"]["ms"])
+        q64.dgq_set_tuning(q, best[0], best[1], best[2], 100000)
+
+    # ---- warmup
+    for _ in range(args.warmup):
+        one_step()
+    q64.dgq_sync(q)
+
+    # ---- timed region: K steps back to back; events bracket each step's sweeps
+    evs = [(eng.event(), eng.event()) for _ in range(args.steps)]
+    eng.sync()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        q64.initZeroState(q)
+        q64.apply_circuit(q, circ)
+        eng.record(a, stream)
+        q64.dgq_flush(q)
+        eng.record(b, stream)
+    q64.dgq_sync(q)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    gpu_ms = sum(eng.elapsed_ms(a, b) for a, b in evs)
+
+    st = q64.plan_stats(q)
+    step_bytes, launches = q64.flush_bytes(q)
+    tp = q64.calcTotalProb(q)
+    if abs(tp - 1.0) > 1e-10:
+        raise RuntimeError(f"total probability drifted: {tp}")
+    total_bytes = step_bytes * args.steps
+    value = total_bytes / wall / 1e9
+    achieved = total_bytes / (gpu_ms * 1e-3) / 1e9
+    q64.destroyQureg(q, env)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded MT19937 circuit)",
+        "config": {
+            "workload": f"{n}-qubit layered random circuit, U3 on every qubit + CZ on a seeded perfect "
+                        f"matching per layer, depth {args.depth}, seed {args.seed}, complex FP64 "
+                        f"({16 << n >> 30} GiB state), plain fused-sweep method",
+            "n_qubits": n,
+            "depth": args.depth,
+            "gates": len(circ),
+            "fusion_k": best[0],
+            "tile_R": best[1],
+            "tile_W": best[2],
+            "sweeps_per_step": st["sweeps"],
+            "lds_exchanges_per_step": st["exchanges"],
+            "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+            "autotune_ms_per_shape": tuning,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(),
+            "kernel": f"sweep_kernel<double,{best[1]},{best[2]}>",
+            "bytes_per_launch": step_bytes / max(1, launches),
+            "avg_launch_ms": round(gpu_ms / (args.steps * max(1, launches)), 4),
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

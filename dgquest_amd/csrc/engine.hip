@@ -17,8 +17,10 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "dgq_engine.h"
@@ -72,100 +74,190 @@ __device__ __forceinline__ uint32_t thread_tile_addr(const dq_phase *__restrict_
 
 // Re-distribute the tile from the layout of phase A to that of phase B
 // through LDS (each thread writes its 2^R amplitudes, then reads 2^R).
-template <int R, int W, typename V>
-__device__ __forceinline__ void exchange(V (&a)[1 << R], V *lds, const dq_phase *__restrict__ A,
+// SPLIT moves the real parts, then the imaginary parts, through an LDS array
+// of scalars: half the LDS footprint (more workgroups per CU), two more
+// barriers.
+template <int R, int W, bool SPLIT, typename V>
+__device__ __forceinline__ void exchange(V (&a)[1 << R], void *lds_raw, const dq_phase *__restrict__ A,
                                          const dq_phase *__restrict__ B, bool lead_barrier, uint32_t tid) {
-    if (lead_barrier) __syncthreads();
-    {
-        const uint32_t sa = swz(thread_tile_addr<W>(A, tid));
-        uint32_t ro[R > 0 ? R : 1];
+    using T = typename std::remove_reference<decltype(a[0].x)>::type;
+    const uint32_t sa = swz(thread_tile_addr<W>(A, tid));
+    const uint32_t sb = swz(thread_tile_addr<W>(B, tid));
+    uint32_t ra[R > 0 ? R : 1], rb[R > 0 ? R : 1];
 #pragma unroll
-        for (int r = 0; r < R; r++) ro[r] = swz(A->reg_t[r]);
+    for (int r = 0; r < R; r++) { ra[r] = swz(A->reg_t[r]); rb[r] = swz(B->reg_t[r]); }
+    // LDS element index of register j: lane/wave part xor the uniform register part
+    auto off = [&](uint32_t s0, const uint32_t *rr, int j) {
+        uint32_t x = 0;
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) {
-            uint32_t o = sa;
+        for (int r = 0; r < R; r++)
+            if ((j >> r) & 1) x ^= rr[r];
+        return s0 ^ x;
+    };
+    if (!SPLIT) {
+        V *lds = reinterpret_cast<V *>(lds_raw);
+        if (lead_barrier) __syncthreads();
 #pragma unroll
-            for (int r = 0; r < R; r++)
-                if ((j >> r) & 1) o ^= ro[r];
-            lds[o] = a[j];
-        }
-    }
-    __syncthreads();
-    {
-        const uint32_t sb = swz(thread_tile_addr<W>(B, tid));
-        uint32_t ro[R > 0 ? R : 1];
+        for (int j = 0; j < (1 << R); j++) lds[off(sa, ra, j)] = a[j];
+        __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; r++) ro[r] = swz(B->reg_t[r]);
+        for (int j = 0; j < (1 << R); j++) a[j] = lds[off(sb, rb, j)];
+    } else {
+        T *lds = reinterpret_cast<T *>(lds_raw);
+        if (lead_barrier) __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) {
-            uint32_t o = sb;
+        for (int j = 0; j < (1 << R); j++) lds[off(sa, ra, j)] = a[j].x;
+        __syncthreads();
+        T re[1 << R];
 #pragma unroll
-            for (int r = 0; r < R; r++)
-                if ((j >> r) & 1) o ^= ro[r];
-            a[j] = lds[o];
-        }
+        for (int j = 0; j < (1 << R); j++) re[j] = lds[off(sb, rb, j)];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < (1 << R); j++) lds[off(sa, ra, j)] = a[j].y;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < (1 << R); j++) { a[j].y = lds[off(sb, rb, j)]; a[j].x = re[j]; }
     }
 }
 
-// 2x2 complex matrix on register bit B.  MASKED: lane predicate `tok` gates
-// the update (control bits held by lanes or waves).
-template <int B, int R, bool MASKED, typename T, typename V>
-__device__ __forceinline__ void op_mat(V (&a)[1 << R], const T *__restrict__ u, uint32_t creg, bool tok) {
-    const T u0r = u[0], u0i = u[1], u1r = u[2], u1i = u[3];
-    const T u2r = u[4], u2i = u[5], u3r = u[6], u3i = u[7];
+// ---- op bodies ---------------------------------------------------------------
+// Every op works in place on the register tile a[].  Conditions on register
+// bits are uniform per register index (scalar branches); conditions on lane or
+// wave bits become per-lane coefficients chosen once per op.
+
+template <typename T> struct SignBits;
+template <> struct SignBits<double> {
+    __device__ static double flip(double x, uint32_t hi_mask) {
+        uint64_t b = __builtin_bit_cast(uint64_t, x);
+        b ^= (uint64_t)hi_mask << 32;
+        return __builtin_bit_cast(double, b);
+    }
+};
+template <> struct SignBits<float> {
+    __device__ static float flip(float x, uint32_t hi_mask) {
+        return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ hi_mask);
+    }
+};
+
+// 2x2 complex matrix on register bit B applied to every pair (no condition).
+template <int B, int R, typename T, typename V>
+__device__ __forceinline__ void op_mat(V (&a)[1 << R], T u0r, T u0i, T u1r, T u1i, T u2r, T u2i, T u3r, T u3i) {
 #pragma unroll
     for (int j0 = 0; j0 < (1 << R); j0++) {
         if (j0 & (1 << B)) continue;
         const int j1 = j0 | (1 << B);
-        if (((uint32_t)j0 & creg) != creg) continue;
         const V x = a[j0], y = a[j1];
         V nx, ny;
         nx.x = u0r * x.x - u0i * x.y + u1r * y.x - u1i * y.y;
         nx.y = u0r * x.y + u0i * x.x + u1r * y.y + u1i * y.x;
         ny.x = u2r * x.x - u2i * x.y + u3r * y.x - u3i * y.y;
         ny.y = u2r * x.y + u2i * x.x + u3r * y.y + u3i * y.x;
-        if (MASKED) {
-            if (tok) { a[j0] = nx; a[j1] = ny; }
-        } else {
-            a[j0] = nx;
-            a[j1] = ny;
-        }
+        a[j0] = nx;
+        a[j1] = ny;
+    }
+}
+
+// Same, only on pairs whose lower register index satisfies the condition.
+template <int B, int R, typename T, typename V>
+__device__ __forceinline__ void op_mat_cond(V (&a)[1 << R], T u0r, T u0i, T u1r, T u1i, T u2r, T u2i, T u3r,
+                                            T u3i, uint32_t creg, uint32_t vreg) {
+#pragma unroll
+    for (int j0 = 0; j0 < (1 << R); j0++) {
+        if (j0 & (1 << B)) continue;
+        if (((uint32_t)j0 & creg) != vreg) continue;
+        const int j1 = j0 | (1 << B);
+        const V x = a[j0], y = a[j1];
+        V nx, ny;
+        nx.x = u0r * x.x - u0i * x.y + u1r * y.x - u1i * y.y;
+        nx.y = u0r * x.y + u0i * x.x + u1r * y.y + u1i * y.x;
+        ny.x = u2r * x.x - u2i * x.y + u3r * y.x - u3i * y.y;
+        ny.y = u2r * x.y + u2i * x.x + u3r * y.y + u3i * y.x;
+        a[j0] = nx;
+        a[j1] = ny;
     }
 }
 
 template <int B, int R, typename V>
-__device__ __forceinline__ void op_swap(V (&a)[1 << R], uint32_t creg, bool tok) {
+__device__ __forceinline__ void op_swap(V (&a)[1 << R], uint32_t creg, uint32_t vreg, bool tok) {
 #pragma unroll
     for (int j0 = 0; j0 < (1 << R); j0++) {
         if (j0 & (1 << B)) continue;
+        if (((uint32_t)j0 & creg) != vreg) continue;
         const int j1 = j0 | (1 << B);
-        if (((uint32_t)j0 & creg) != creg) continue;
         const V x = a[j0], y = a[j1];
         a[j0] = tok ? y : x;
         a[j1] = tok ? x : y;
     }
 }
 
-template <int R, bool MASKED, typename T, typename V>
-__device__ __forceinline__ void dispatch_mat(V (&a)[1 << R], uint32_t tb, const T *u, uint32_t creg, bool tok) {
+template <int R, typename T, typename V>
+__device__ __forceinline__ void dispatch_mat(V (&a)[1 << R], uint32_t tb, const T *__restrict__ u, uint32_t creg,
+                                             uint32_t vreg, bool lane_cond, bool tok) {
+    T c0r = u[0], c0i = u[1], c1r = u[2], c1i = u[3], c2r = u[4], c2i = u[5], c3r = u[6], c3i = u[7];
+    if (lane_cond) {  // lanes failing the condition apply the identity
+        c0r = tok ? c0r : T(1); c0i = tok ? c0i : T(0); c1r = tok ? c1r : T(0); c1i = tok ? c1i : T(0);
+        c2r = tok ? c2r : T(0); c2i = tok ? c2i : T(0); c3r = tok ? c3r : T(1); c3i = tok ? c3i : T(0);
+    }
+    if (creg == 0) {
+        switch (tb) {
+            case 0: if constexpr (R > 0) op_mat<0, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
+            case 1: if constexpr (R > 1) op_mat<1, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
+            case 2: if constexpr (R > 2) op_mat<2, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
+            case 3: if constexpr (R > 3) op_mat<3, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
+            case 4: if constexpr (R > 4) op_mat<4, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
+            default: break;
+        }
+    } else {
+        switch (tb) {
+            case 0: if constexpr (R > 0) op_mat_cond<0, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
+            case 1: if constexpr (R > 1) op_mat_cond<1, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
+            case 2: if constexpr (R > 2) op_mat_cond<2, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
+            case 3: if constexpr (R > 3) op_mat_cond<3, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
+            case 4: if constexpr (R > 4) op_mat_cond<4, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
+            default: break;
+        }
+    }
+}
+
+// real 2x2 matrix on register bit B: 8 FP64 (or FP32) operations per pair
+template <int B, int R, typename T, typename V>
+__device__ __forceinline__ void op_rmat(V (&a)[1 << R], T m0, T m1, T m2, T m3) {
+#pragma unroll
+    for (int j0 = 0; j0 < (1 << R); j0++) {
+        if (j0 & (1 << B)) continue;
+        const int j1 = j0 | (1 << B);
+        const V x = a[j0], y = a[j1];
+        V nx, ny;
+        nx.x = m0 * x.x + m1 * y.x;
+        nx.y = m0 * x.y + m1 * y.y;
+        ny.x = m2 * x.x + m3 * y.x;
+        ny.y = m2 * x.y + m3 * y.y;
+        a[j0] = nx;
+        a[j1] = ny;
+    }
+}
+
+template <int R, typename T, typename V>
+__device__ __forceinline__ void dispatch_rmat(V (&a)[1 << R], uint32_t tb, const T *__restrict__ m) {
+    const T m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
     switch (tb) {
-        case 0: if constexpr (R > 0) op_mat<0, R, MASKED, T>(a, u, creg, tok); break;
-        case 1: if constexpr (R > 1) op_mat<1, R, MASKED, T>(a, u, creg, tok); break;
-        case 2: if constexpr (R > 2) op_mat<2, R, MASKED, T>(a, u, creg, tok); break;
-        case 3: if constexpr (R > 3) op_mat<3, R, MASKED, T>(a, u, creg, tok); break;
-        case 4: if constexpr (R > 4) op_mat<4, R, MASKED, T>(a, u, creg, tok); break;
+        case 0: if constexpr (R > 0) op_rmat<0, R>(a, m0, m1, m2, m3); break;
+        case 1: if constexpr (R > 1) op_rmat<1, R>(a, m0, m1, m2, m3); break;
+        case 2: if constexpr (R > 2) op_rmat<2, R>(a, m0, m1, m2, m3); break;
+        case 3: if constexpr (R > 3) op_rmat<3, R>(a, m0, m1, m2, m3); break;
+        case 4: if constexpr (R > 4) op_rmat<4, R>(a, m0, m1, m2, m3); break;
         default: break;
     }
 }
 
 template <int R, typename V>
-__device__ __forceinline__ void dispatch_swap(V (&a)[1 << R], uint32_t tb, uint32_t creg, bool tok) {
+__device__ __forceinline__ void dispatch_swap(V (&a)[1 << R], uint32_t tb, uint32_t creg, uint32_t vreg, bool tok) {
     switch (tb) {
-        case 0: if constexpr (R > 0) op_swap<0, R>(a, creg, tok); break;
-        case 1: if constexpr (R > 1) op_swap<1, R>(a, creg, tok); break;
-        case 2: if constexpr (R > 2) op_swap<2, R>(a, creg, tok); break;
-        case 3: if constexpr (R > 3) op_swap<3, R>(a, creg, tok); break;
-        case 4: if constexpr (R > 4) op_swap<4, R>(a, creg, tok); break;
+        case 0: if constexpr (R > 0) op_swap<0, R>(a, creg, vreg, tok); break;
+        case 1: if constexpr (R > 1) op_swap<1, R>(a, creg, vreg, tok); break;
+        case 2: if constexpr (R > 2) op_swap<2, R>(a, creg, vreg, tok); break;
+        case 3: if constexpr (R > 3) op_swap<3, R>(a, creg, vreg, tok); break;
+        case 4: if constexpr (R > 4) op_swap<4, R>(a, creg, vreg, tok); break;
         default: break;
     }
 }
@@ -175,69 +267,74 @@ __device__ __forceinline__ void run_ops(V (&a)[1 << R], const dq_op *__restrict_
                                         uint64_t base, uint32_t tid) {
     for (uint32_t k = 0; k < count; k++) {
         const dq_op *__restrict__ op = ops + k;
-        const uint64_t cwg = op->c_wg;
-        if ((base & cwg) != cwg) continue;
+        if ((base & op->c_wg) != op->v_wg) continue;
         const uint32_t kind = op->kind;
-        const uint32_t creg = op->c_reg;
+        const uint32_t creg = op->c_reg, vreg = op->v_reg;
         const uint32_t cthr = op->c_thr;
-        const bool tok = (tid & cthr) == cthr;
+        const bool tok = (tid & cthr) == op->v_thr;
         const T *__restrict__ m = reinterpret_cast<const T *>(op->m);
         if (kind == DQ_OP_MAT) {
-            if (cthr == 0) dispatch_mat<R, false, T>(a, op->tbit, m, creg, true);
-            else dispatch_mat<R, true, T>(a, op->tbit, m, creg, tok);
+            dispatch_mat<R, T>(a, op->tbit, m, creg, vreg, cthr != 0, tok);
         } else if (kind == DQ_OP_NEG) {
+            const uint32_t hm = (cthr == 0 || tok) ? (sizeof(T) == 8 ? 0x80000000u : 0x80000000u) : 0u;
 #pragma unroll
             for (int j = 0; j < (1 << R); j++) {
-                if (((uint32_t)j & creg) != creg) continue;
-                V x = a[j];
-                a[j].x = tok ? -x.x : x.x;
-                a[j].y = tok ? -x.y : x.y;
+                if (((uint32_t)j & creg) != vreg) continue;
+                a[j].x = SignBits<T>::flip(a[j].x, hm);
+                a[j].y = SignBits<T>::flip(a[j].y, hm);
             }
         } else if (kind == DQ_OP_PHASE) {
-            const T pr = m[0], pi = m[1];
+            T pr = m[0], pi = m[1];
+            if (cthr != 0) { pr = tok ? pr : T(1); pi = tok ? pi : T(0); }
 #pragma unroll
             for (int j = 0; j < (1 << R); j++) {
-                if (((uint32_t)j & creg) != creg) continue;
-                V x = a[j];
-                V y;
-                y.x = pr * x.x - pi * x.y;
-                y.y = pr * x.y + pi * x.x;
-                a[j] = tok ? y : x;
-            }
-        } else if (kind == DQ_OP_DIAG2) {
-            const uint32_t sreg = op->s_reg, sthr = op->s_thr;
-            const uint64_t swg = op->s_wg;
-            const bool sel_t = ((tid & sthr) != 0) || ((base & swg) != 0);
-            const T p0r = m[0], p0i = m[1], p1r = m[2], p1i = m[3];
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) {
-                if (((uint32_t)j & creg) != creg) continue;
-                const bool sel = sel_t || (((uint32_t)j & sreg) != 0);
-                const T pr = sel ? p1r : p0r, pi = sel ? p1i : p0i;
-                V x = a[j];
-                V y;
-                y.x = pr * x.x - pi * x.y;
-                y.y = pr * x.y + pi * x.x;
-                a[j] = tok ? y : x;
+                if (((uint32_t)j & creg) != vreg) continue;
+                const V x = a[j];
+                a[j].x = pr * x.x - pi * x.y;
+                a[j].y = pr * x.y + pi * x.x;
             }
         } else if (kind == DQ_OP_SWAP) {
-            dispatch_swap<R>(a, op->tbit, creg, tok);
+            dispatch_swap<R>(a, op->tbit, creg, vreg, cthr == 0 || tok);
+        } else if (kind == DQ_OP_RMAT) {
+            dispatch_rmat<R, T>(a, op->tbit, m);
+        } else if (kind == DQ_OP_DIAGR) {
+            // per-register-index phases, uniform across the workgroup
+#pragma unroll
+            for (int j = 0; j < (1 << R); j++) {
+                const T *__restrict__ e = reinterpret_cast<const T *>(op[1 + (j >> 2)].m) + 2 * (j & 3);
+                const T pr = e[0], pi = e[1];
+                const V x = a[j];
+                a[j].x = pr * x.x - pi * x.y;
+                a[j].y = pr * x.y + pi * x.x;
+            }
+            k += op->tbit;
         }
     }
 }
 
-template <typename T, int R, int W>
-__global__ void __launch_bounds__(64 << W)
+// Occupancy target (waves per SIMD) of a sweep instantiation: the register
+// tile takes 2^R amplitudes x (2 * sizeof(T) / 4) VGPRs; allow about as many
+// again (at least 48) for addresses, matrix entries and temporaries.
+template <typename T, int R>
+constexpr int sweep_waves_per_eu() {
+    constexpr int amp = (1 << R) * (int)(2 * sizeof(T) / 4);
+    constexpr int budget = amp + (amp > 48 ? amp : 48) + 24;
+    constexpr int w = 512 / budget;
+    return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+template <typename T, int R, int W, bool SPLIT>
+__global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R>()))
 sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
              const dq_phase *__restrict__ phases, const dq_op *__restrict__ ops) {
     using V = typename V2<T>::type;
+    using GV = __attribute__((address_space(1))) V;
     constexpr int NR = 1 << R;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    V *lds = reinterpret_cast<V *>(lds_raw);
 
     const dq_job job = jobs[blockIdx.y];
     const dq_block *__restrict__ blk = blocks + job.block;
-    V *__restrict__ st = reinterpret_cast<V *>(job.state);
+    GV *__restrict__ st = (GV *)(job.state);
     const uint32_t tid = threadIdx.x;
 
     // non-tile qubits come from the workgroup index
@@ -257,10 +354,14 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
     const uint32_t nph = blk->phase_count;
     V a[NR];
 
-    {   // load: lane l = qubit l, register / wave bits = ld_reg / ld_wave
-        uint64_t tb = base | (tid & 63u);
+    {   // load: lane l = qubit l, register / wave bits = ld_reg / ld_wave.
+        // Address = uniform part (tile base | register bits) + per-lane part.
+        uint32_t toff = tid & 63u;
 #pragma unroll
-        for (int w = 0; w < W; w++) tb |= ((tid >> (6 + w)) & 1u) ? blk->ld_wave[w] : 0ull;
+        for (int w = 0; w < W; w++) toff |= ((tid >> (6 + w)) & 1u) ? (uint32_t)blk->ld_wave[w] : 0u;
+        uint64_t whi = 0;  // wave bits above bit 31 go to the uniform part per wave
+#pragma unroll
+        for (int w = 0; w < W; w++) whi |= ((tid >> (6 + w)) & 1u) ? (blk->ld_wave[w] & ~0xFFFFFFFFull) : 0ull;
         uint64_t qo[R > 0 ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; r++) qo[r] = blk->ld_reg[r];
@@ -268,21 +369,22 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
         if (ik == DQ_INIT_LOAD) {
 #pragma unroll
             for (int j = 0; j < NR; j++) {
-                uint64_t idx = tb;
+                uint64_t u = base;
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    if ((j >> r) & 1) idx |= qo[r];
-                a[j] = __builtin_nontemporal_load(st + idx);
+                    if ((j >> r) & 1) u |= qo[r];
+                const GV *p = st + (u | whi);
+                a[j] = __builtin_nontemporal_load(p + toff);
             }
         } else if (ik == DQ_INIT_CLASSICAL) {
             const uint64_t want = blk->init_index;
 #pragma unroll
             for (int j = 0; j < NR; j++) {
-                uint64_t idx = tb;
+                uint64_t u = base | whi | toff;
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    if ((j >> r) & 1) idx |= qo[r];
-                a[j].x = (idx == want) ? T(1) : T(0);
+                    if ((j >> r) & 1) u |= qo[r];
+                a[j].x = (u == want) ? T(1) : T(0);
                 a[j].y = T(0);
             }
         } else {
@@ -293,31 +395,172 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
     }
 
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W>(a, lds, ph + (p - 1), ph + p, p > 1, tid);
+        if (p > 0) exchange<R, W, SPLIT>(a, (void *)lds_raw, ph + (p - 1), ph + p, p > 1, tid);
         run_ops<R, T>(a, ops + ph[p].op_begin, ph[p].op_count, base, tid);
     }
 
     {   // store: lane l = qubit l, register / wave bits = st_reg / st_wave
-        uint64_t tb = base | (tid & 63u);
+        uint32_t toff = tid & 63u;
 #pragma unroll
-        for (int w = 0; w < W; w++) tb |= ((tid >> (6 + w)) & 1u) ? blk->st_wave[w] : 0ull;
+        for (int w = 0; w < W; w++) toff |= ((tid >> (6 + w)) & 1u) ? (uint32_t)blk->st_wave[w] : 0u;
+        uint64_t whi = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) whi |= ((tid >> (6 + w)) & 1u) ? (blk->st_wave[w] & ~0xFFFFFFFFull) : 0ull;
         uint64_t qo[R > 0 ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; r++) qo[r] = blk->st_reg[r];
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            uint64_t idx = tb;
+            uint64_t u = base;
 #pragma unroll
             for (int r = 0; r < R; r++)
-                if ((j >> r) & 1) idx |= qo[r];
-            __builtin_nontemporal_store(a[j], st + idx);
+                if ((j >> r) & 1) u |= qo[r];
+            GV *p = st + (u | whi);
+            __builtin_nontemporal_store(a[j], p + toff);
         }
     }
 }
 
-// Small registers (n <= 12): whole state in LDS, one workgroup per job.  Ops
-// are in global-index form: tbit = target qubit, c_wg = control mask over the
-// full index, s_wg = selector bit over the full index.
+// ---- persistent, software-pipelined sweep -----------------------------------
+// One job (one state) per launch; the grid is sized to what stays resident and
+// each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...  The
+// loads of the next tile are issued into a second register set before the
+// current tile is computed and stored, so every wave keeps HBM busy while its
+// VALU and LDS work runs.
+template <typename T, int R, int W>
+struct TileIO {
+    using V = typename V2<T>::type;
+    using GV = __attribute__((address_space(1))) V;
+
+    __device__ static uint64_t tile_base(const dq_block *__restrict__ blk, uint64_t t) {
+        uint64_t base = 0;
+        uint64_t msk = blk->nontile_mask;
+        while (msk) {
+            const int p = __builtin_ctzll(msk);
+            base |= (t & 1ull) << p;
+            t >>= 1;
+            msk &= msk - 1;
+        }
+        return base;
+    }
+
+    __device__ static void load(V (&a)[1 << R], GV *__restrict__ st, const dq_block *__restrict__ blk,
+                                uint64_t base, uint32_t tid) {
+        uint32_t toff = tid & 63u;
+        uint64_t whi = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const bool on = (tid >> (6 + w)) & 1u;
+            toff |= on ? (uint32_t)blk->ld_wave[w] : 0u;
+            whi |= on ? (blk->ld_wave[w] & ~0xFFFFFFFFull) : 0ull;
+        }
+        const uint32_t ik = blk->init_kind;
+        if (ik == DQ_INIT_LOAD) {
+#pragma unroll
+            for (int j = 0; j < (1 << R); j++) {
+                uint64_t u = base;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if ((j >> r) & 1) u |= blk->ld_reg[r];
+                a[j] = __builtin_nontemporal_load(st + (u | whi) + toff);
+            }
+        } else if (ik == DQ_INIT_CLASSICAL) {
+            const uint64_t want = blk->init_index;
+#pragma unroll
+            for (int j = 0; j < (1 << R); j++) {
+                uint64_t u = base | whi | toff;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if ((j >> r) & 1) u |= blk->ld_reg[r];
+                a[j].x = (u == want) ? T(1) : T(0);
+                a[j].y = T(0);
+            }
+        } else {
+            const T v = (T)blk->init_value;
+#pragma unroll
+            for (int j = 0; j < (1 << R); j++) { a[j].x = v; a[j].y = T(0); }
+        }
+    }
+
+    __device__ static void store(V (&a)[1 << R], GV *__restrict__ st, const dq_block *__restrict__ blk,
+                                 uint64_t base, uint32_t tid) {
+        uint32_t toff = tid & 63u;
+        uint64_t whi = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const bool on = (tid >> (6 + w)) & 1u;
+            toff |= on ? (uint32_t)blk->st_wave[w] : 0u;
+            whi |= on ? (blk->st_wave[w] & ~0xFFFFFFFFull) : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < (1 << R); j++) {
+            uint64_t u = base;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if ((j >> r) & 1) u |= blk->st_reg[r];
+            __builtin_nontemporal_store(a[j], st + (u | whi) + toff);
+        }
+    }
+};
+
+template <typename T, int R, int W, bool SPLIT>
+__device__ __forceinline__ void compute_tile(typename V2<T>::type (&a)[1 << R], void *lds,
+                                             const dq_phase *__restrict__ ph, uint32_t nph,
+                                             const dq_op *__restrict__ ops, uint64_t base, uint32_t tid) {
+    for (uint32_t p = 0; p < nph; p++) {
+        // every exchange starts with a barrier: the previous tile's last
+        // exchange may still be read by other waves
+        if (p > 0) exchange<R, W, SPLIT>(a, lds, ph + (p - 1), ph + p, true, tid);
+        run_ops<R, T>(a, ops + ph[p].op_begin, ph[p].op_count, base, tid);
+    }
+}
+
+template <typename T, int R, int W, bool SPLIT>
+__global__ void __launch_bounds__(64 << W, 1)
+sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
+                  const dq_phase *__restrict__ phases, const dq_op *__restrict__ ops, uint64_t ntiles) {
+    using V = typename V2<T>::type;
+    using IO = TileIO<T, R, W>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const dq_job job = jobs[0];
+    const dq_block *__restrict__ blk = blocks + job.block;
+    typename IO::GV *__restrict__ st = (typename IO::GV *)(job.state);
+    const uint32_t tid = threadIdx.x;
+    const dq_phase *__restrict__ ph = phases + blk->phase_begin;
+    const uint32_t nph = blk->phase_count;
+
+    V a[1 << R], b[1 << R];
+    uint64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    uint64_t base_t = IO::tile_base(blk, t);
+    IO::load(a, st, blk, base_t, tid);
+    while (true) {
+        const uint64_t t2 = t + gridDim.x;
+        uint64_t base2 = 0;
+        if (t2 < ntiles) {
+            base2 = IO::tile_base(blk, t2);
+            IO::load(b, st, blk, base2, tid);
+        }
+        compute_tile<T, R, W, SPLIT>(a, (void *)lds_raw, ph, nph, ops, base_t, tid);
+        IO::store(a, st, blk, base_t, tid);
+        if (t2 >= ntiles) break;
+        const uint64_t t3 = t2 + gridDim.x;
+        uint64_t base3 = 0;
+        if (t3 < ntiles) {
+            base3 = IO::tile_base(blk, t3);
+            IO::load(a, st, blk, base3, tid);
+        }
+        compute_tile<T, R, W, SPLIT>(b, (void *)lds_raw, ph, nph, ops, base2, tid);
+        IO::store(b, st, blk, base2, tid);
+        if (t3 >= ntiles) break;
+        t = t3;
+        base_t = base3;
+    }
+}
+
+// Small registers (n <= 13): whole state in LDS, one workgroup per job.  Ops
+// are in global-index form: tbit = target qubit, (c_wg, v_wg) = condition
+// over the full index.
 template <typename T>
 __global__ void __launch_bounds__(256)
 small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
@@ -341,17 +584,23 @@ small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
         for (uint32_t k = 0; k < ph[p].op_count; k++) {
             const dq_op *__restrict__ op = ops + ph[p].op_begin + k;
             const T *__restrict__ m = reinterpret_cast<const T *>(op->m);
-            const uint64_t c = op->c_wg;
+            const uint64_t c = op->c_wg, v = op->v_wg;
             __syncthreads();
-            if (op->kind == DQ_OP_MAT || op->kind == DQ_OP_SWAP) {
+            if (op->kind == DQ_OP_MAT || op->kind == DQ_OP_SWAP || op->kind == DQ_OP_RMAT) {
                 const uint32_t t = op->tbit, tm = 1u << t;
                 for (uint32_t k2 = tid; k2 < N / 2; k2 += blockDim.x) {
                     const uint32_t i0 = ((k2 >> t) << (t + 1)) | (k2 & (tm - 1));
                     const uint32_t i1 = i0 | tm;
-                    if ((i0 & c) != c) continue;
+                    if ((i0 & c) != v) continue;
                     const V x = s[i0], y = s[i1];
                     if (op->kind == DQ_OP_SWAP) { s[i0] = y; s[i1] = x; continue; }
                     V nx, ny;
+                    if (op->kind == DQ_OP_RMAT) {
+                        nx.x = m[0] * x.x + m[1] * y.x; nx.y = m[0] * x.y + m[1] * y.y;
+                        ny.x = m[2] * x.x + m[3] * y.x; ny.y = m[2] * x.y + m[3] * y.y;
+                        s[i0] = nx; s[i1] = ny;
+                        continue;
+                    }
                     nx.x = m[0] * x.x - m[1] * x.y + m[2] * y.x - m[3] * y.y;
                     nx.y = m[0] * x.y + m[1] * x.x + m[2] * y.y + m[3] * y.x;
                     ny.x = m[4] * x.x - m[5] * x.y + m[6] * y.x - m[7] * y.y;
@@ -361,14 +610,12 @@ small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
                 }
             } else {
                 for (uint32_t i = tid; i < N; i += blockDim.x) {
-                    if ((i & c) != c) continue;
+                    if ((i & c) != v) continue;
                     V x = s[i];
                     if (op->kind == DQ_OP_NEG) { x.x = -x.x; x.y = -x.y; s[i] = x; continue; }
-                    T pr = m[0], pi = m[1];
-                    if (op->kind == DQ_OP_DIAG2 && (i & op->s_wg)) { pr = m[2]; pi = m[3]; }
                     V y;
-                    y.x = pr * x.x - pi * x.y;
-                    y.y = pr * x.y + pi * x.x;
+                    y.x = m[0] * x.x - m[1] * x.y;
+                    y.y = m[0] * x.y + m[1] * x.x;
                     s[i] = y;
                 }
             }
@@ -499,12 +746,20 @@ __global__ void fill_kernel(typename V2<T>::type *__restrict__ st, uint64_t N, i
 // ----------------------------------------------------------------------------
 typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, const dq_op *);
 
+typedef void (*pipe_fn)(const dq_job *, const dq_block *, const dq_phase *, const dq_op *, uint64_t);
+
 template <typename T>
 struct SweepTable {
-    sweep_fn fn[DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
+    sweep_fn fn[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
+    pipe_fn pipe[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
     SweepTable() {
         memset(fn, 0, sizeof fn);
-#define DQ_INST(R_, W_) fn[R_][W_] = sweep_kernel<T, R_, W_>;
+        memset(pipe, 0, sizeof pipe);
+#define DQ_INST(R_, W_)                                                                  \
+    fn[0][R_][W_] = sweep_kernel<T, R_, W_, false>;                                      \
+    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;                                       \
+    pipe[0][R_][W_] = sweep_pipe_kernel<T, R_, W_, false>;                               \
+    pipe[1][R_][W_] = sweep_pipe_kernel<T, R_, W_, true>;
         DQ_INST(1, 0) DQ_INST(2, 0) DQ_INST(3, 0) DQ_INST(4, 0) DQ_INST(5, 0)
         DQ_INST(2, 1) DQ_INST(3, 1) DQ_INST(4, 1) DQ_INST(5, 1)
         DQ_INST(2, 2) DQ_INST(3, 2) DQ_INST(4, 2) DQ_INST(5, 2)
@@ -516,9 +771,36 @@ struct SweepTable {
 static SweepTable<double> g_tab64;
 static SweepTable<float> g_tab32;
 
-static sweep_fn lookup(int prec, int R, int W) {
+// split exchanges: on for FP64 tiles of 12+ bits unless DGQ_SPLIT_EXCHANGE=0
+static int g_split_mode = -1;
+static bool use_split(int prec, int T) {
+    if (g_split_mode < 0) {
+        const char *e = getenv("DGQ_SPLIT_EXCHANGE");
+        g_split_mode = e ? atoi(e) : 2;
+    }
+    if (g_split_mode == 0) return false;
+    if (g_split_mode == 1) return true;
+    return prec == DQ_F64 && T >= 12;
+}
+
+// persistent pipelined kernel for single-state launches (DGQ_PIPE=1 enables)
+static int g_pipe_mode = -1;
+static bool use_pipe() {
+    if (g_pipe_mode < 0) {
+        const char *e = getenv("DGQ_PIPE");
+        g_pipe_mode = e ? atoi(e) : 0;
+    }
+    return g_pipe_mode != 0;
+}
+
+static pipe_fn lookup_pipe(int prec, int R, int W, bool split) {
     if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS) return nullptr;
-    return prec == DQ_F64 ? g_tab64.fn[R][W] : g_tab32.fn[R][W];
+    return prec == DQ_F64 ? g_tab64.pipe[split][R][W] : g_tab32.pipe[split][R][W];
+}
+
+static sweep_fn lookup(int prec, int R, int W, bool split = false) {
+    if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS) return nullptr;
+    return prec == DQ_F64 ? g_tab64.fn[split][R][W] : g_tab32.fn[split][R][W];
 }
 
 static size_t elem_bytes(int prec) { return prec == DQ_F64 ? 16 : 8; }
@@ -536,6 +818,7 @@ std::mutex g_stage_mu;
 StageSlot g_stage[16];
 unsigned g_stage_next = 0;
 int g_device = -1;
+int g_num_cus = 256;
 }  // namespace
 
 static int stage_upload(void *d_dst, const std::vector<unsigned char> &src, void *stream) {
@@ -563,13 +846,23 @@ extern "C" int dq_device_init(int device) {
     HIPC(hipSetDevice(device));
     if (g_device != device) {
         // allow the large LDS tiles (up to 128 KiB) on every sweep instantiation
-        for (int R = 0; R <= DQ_MAX_REG_BITS; R++)
-            for (int W = 0; W <= DQ_MAX_WAVE_BITS; W++) {
-                if (g_tab64.fn[R][W])
-                    HIPC(hipFuncSetAttribute((const void *)g_tab64.fn[R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-                if (g_tab32.fn[R][W])
-                    HIPC(hipFuncSetAttribute((const void *)g_tab32.fn[R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-            }
+        for (int sp = 0; sp < 2; sp++)
+            for (int R = 0; R <= DQ_MAX_REG_BITS; R++)
+                for (int W = 0; W <= DQ_MAX_WAVE_BITS; W++) {
+                    if (g_tab64.fn[sp][R][W])
+                        HIPC(hipFuncSetAttribute((const void *)g_tab64.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                    if (g_tab32.fn[sp][R][W])
+                        HIPC(hipFuncSetAttribute((const void *)g_tab32.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                    if (g_tab64.pipe[sp][R][W])
+                        HIPC(hipFuncSetAttribute((const void *)g_tab64.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                    if (g_tab32.pipe[sp][R][W])
+                        HIPC(hipFuncSetAttribute((const void *)g_tab32.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                }
+        {
+            hipDeviceProp_t prop;
+            HIPC(hipGetDeviceProperties(&prop, device));
+            g_num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        }
         HIPC(hipFuncSetAttribute((const void *)small_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
         HIPC(hipFuncSetAttribute((const void *)small_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
         g_device = device;
@@ -668,7 +961,7 @@ static bool onehot64(uint64_t x) { return x && !(x & (x - 1)); }
 // Host-side check of every descriptor against the grid and buffers it
 // implies: each workgroup reads and writes exactly the 2^T amplitudes of its
 // own tile, every LDS index is below 2^T, every op index is in range.
-static int validate_block(const dq_block &b, const dq_phase *phases, uint32_t n_phases, uint32_t n_ops) {
+static int validate_block(const dq_block &b, const dq_phase *phases, uint32_t n_phases, const dq_op *ops, uint32_t n_ops) {
     const int n = (int)b.n_qubits;
     if (n < 1 || n > 40) return fail(DQ_ERR_ARG, "block: n_qubits %d out of range", n);
     if (b.phase_count < 1 || b.phase_begin >= n_phases || b.phase_count > n_phases - b.phase_begin)
@@ -676,6 +969,19 @@ static int validate_block(const dq_block &b, const dq_phase *phases, uint32_t n_
     for (uint32_t p = 0; p < b.phase_count; p++) {
         const dq_phase &ph = phases[b.phase_begin + p];
         if (ph.op_begin > n_ops || ph.op_count > n_ops - ph.op_begin) return fail(DQ_ERR_ARG, "phase: op range out of bounds");
+        for (uint32_t k = 0; k < ph.op_count; k++) {
+            const dq_op &o = ops[ph.op_begin + k];
+            if (o.kind == DQ_OP_DIAGR) {
+                if (b.reg_bits == 0 || o.tbit != ((1u << b.reg_bits) + 3) / 4 || k + o.tbit >= ph.op_count)
+                    return fail(DQ_ERR_ARG, "phase: register-diagonal table out of bounds");
+                k += o.tbit;
+            } else if (o.kind == DQ_OP_MAT || o.kind == DQ_OP_SWAP || o.kind == DQ_OP_RMAT) {
+                if (b.reg_bits ? o.tbit >= b.reg_bits : o.tbit >= b.n_qubits)
+                    return fail(DQ_ERR_ARG, "op: target bit %u out of range", o.tbit);
+            } else if (o.kind != DQ_OP_NEG && o.kind != DQ_OP_PHASE) {
+                return fail(DQ_ERR_ARG, "op: unknown kind %u", o.kind);
+            }
+        }
     }
     if (b.reg_bits == 0) {  // small-state form
         if ((int)b.tile_bits != n || n > 13) return fail(DQ_ERR_ARG, "small block: tile_bits must equal n <= 13");
@@ -720,7 +1026,7 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
     const size_t need = dq_apply_ws_bytes(n_blocks, n_phases, n_ops, n_jobs);
     if (need > ws_bytes) return fail(DQ_ERR_ARG, "dq_apply_blocks: workspace %zu < %zu bytes", ws_bytes, need);
     for (uint32_t i = 0; i < n_blocks; i++) {
-        int rc = validate_block(blocks[i], phases, n_phases, n_ops);
+        int rc = validate_block(blocks[i], phases, n_phases, ops, n_ops);
         if (rc) return rc;
     }
     for (uint32_t j = 0; j < n_jobs; j++)
@@ -767,13 +1073,25 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
         } else {
             bool multi = false;
             for (uint32_t j = j0; j < j0 + nj; j++) multi |= blocks[jobs[j].block].phase_count > 1;
-            const size_t lds = multi ? ((size_t)1 << b0.tile_bits) * elem_bytes(prec) : 0;
+            const bool split = use_split(prec, b0.tile_bits);
+            const size_t lds = multi ? ((size_t)1 << b0.tile_bits) * (split ? elem_bytes(prec) / 2 : elem_bytes(prec)) : 0;
             if (lds > 131072) return fail(DQ_ERR_UNSUPPORTED, "tile of %d bits needs %zu bytes of LDS", b0.tile_bits, lds);
             const uint64_t tiles = 1ull << (n - b0.tile_bits);
             if (tiles > 0x7fffffffull || nj > 65535) return fail(DQ_ERR_UNSUPPORTED, "grid too large");
-            sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits);
-            hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                               d_jobs + j0, d_blocks, d_phases, d_ops);
+            if (nj == 1 && use_pipe()) {
+                pipe_fn pf = lookup_pipe(prec, b0.reg_bits, b0.wave_bits, split);
+                int per_cu = 0;
+                HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pf, 64 << b0.wave_bits, lds));
+                if (per_cu < 1) per_cu = 1;
+                uint64_t grid = (uint64_t)per_cu * (uint64_t)g_num_cus;
+                if (grid > tiles) grid = tiles;
+                hipLaunchKernelGGL(pf, dim3((unsigned)grid), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
+                                   d_jobs + j0, d_blocks, d_phases, d_ops, tiles);
+            } else {
+                sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
+                hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
+                                   d_jobs + j0, d_blocks, d_phases, d_ops);
+            }
         }
         HIPC(hipGetLastError());
     }

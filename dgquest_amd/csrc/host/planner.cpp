@@ -2,7 +2,9 @@
 #include "planner.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <complex>
 #include <cstring>
 
 #include "dag.hpp"
@@ -78,80 +80,304 @@ void split_mask(const TileCtx &c, uint64_t qmask, uint32_t &reg, uint32_t &thr, 
     }
 }
 
-// Device op(s) for one gate in the current phase layout (or, for the small
-// form, in global-index form).  Returns false for a gate that needs nothing.
-bool pack_gate(const Gate &g, const TileCtx &c, bool small, dq_op &o) {
+using cd = std::complex<double>;
+
+// Ops of one phase with double coefficients; rounded to the state precision
+// when appended to the plan.
+struct OpList {
+    std::vector<dq_op> ops;
+    std::vector<std::array<double, 8>> coef;
+    void add(uint32_t kind, uint32_t tbit, const dq_op &cond, const double *m, int nm) {
+        dq_op o = cond;
+        o.kind = kind;
+        o.tbit = tbit;
+        std::array<double, 8> a{};
+        for (int i = 0; i < nm; i++) a[i] = m[i];
+        ops.push_back(o);
+        coef.push_back(a);
+    }
+    void flush_to(std::vector<dq_op> &out, int prec) {
+        for (size_t i = 0; i < ops.size(); i++) {
+            dq_op o = ops[i];
+            for (int e = 0; e < 8; e++) put(o, e, coef[i][e], prec);
+            out.push_back(o);
+        }
+        ops.clear();
+        coef.clear();
+    }
+};
+
+dq_op cond_of(const TileCtx &c, bool small, uint64_t cmask, uint64_t cval) {
+    dq_op o;
     memset(&o, 0, sizeof o);
-    const int prec = c.prec;
-    auto ctrl = [&](uint64_t qm) {
-        if (small) o.c_wg = qm;
-        else split_mask(c, qm, o.c_reg, o.c_thr, o.c_wg);
-    };
-    auto sel = [&](uint64_t qm) {
-        if (small) o.s_wg = qm;
-        else split_mask(c, qm, o.s_reg, o.s_thr, o.s_wg);
-    };
-    auto target = [&](uint64_t qm) -> uint32_t {
-        const int q = ctz64(qm);
-        if (small) return (uint32_t)q;
-        const int b = c.q2t[q];
-        return (uint32_t)c.roles.idx[b];  // planner guarantees a register role
-    };
+    if (small) {
+        o.c_wg = cmask;
+        o.v_wg = cval;
+    } else {
+        split_mask(c, cmask, o.c_reg, o.c_thr, o.c_wg);
+        split_mask(c, cval, o.v_reg, o.v_thr, o.v_wg);
+    }
+    return o;
+}
+
+// register bit holding qubit q in the current layout, or -1
+int reg_of(const TileCtx &c, int q) {
+    const int b = c.q2t[q];
+    if (b < 0 || c.roles.kind[b] != 0) return -1;
+    return c.roles.idx[b];
+}
+
+// Diagonal fusion state of one flush.  The device state equals the logical
+// state times prod_q diag(p0[q], p1[q]) times g: single-qubit diagonals are
+// kept here (they commute with every diagonal and with controls) until the
+// next non-diagonal gate on their qubit absorbs them.
+struct Fuse {
+    bool on = false;
+    std::vector<cd> p0, p1;
+    cd g = 1.0;
+    explicit Fuse(int n = 0) : p0(n, 1.0), p1(n, 1.0) {}
+    bool trivial(int q) const { return p0[q] == cd(1.0) && p1[q] == cd(1.0); }
+};
+
+// Pending real rotations of one phase: their right diagonals diag(1, r1) are
+// applied together as one register-indexed diagonal, then the rotations.
+struct Batch {
+    std::vector<int> reg;
+    std::vector<std::array<double, 4>> M;
+    std::vector<cd> r1;
+    uint64_t qmask = 0;
+};
+
+void emit_table(OpList &L, const TileCtx &c, const std::vector<cd> &table) {
+    const int n_ent = 1 << c.R;
+    const int n_cont = (n_ent + 3) / 4;
+    dq_op z;
+    memset(&z, 0, sizeof z);
+    L.add(DQ_OP_DIAGR, (uint32_t)n_cont, z, nullptr, 0);
+    for (int k = 0; k < n_cont; k++) {
+        double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int e = 0; e < 4 && 4 * k + e < n_ent; e++) {
+            m[2 * e] = table[4 * k + e].real();
+            m[2 * e + 1] = table[4 * k + e].imag();
+        }
+        L.add(0, 0, z, m, 8);
+    }
+}
+
+void flush_batch(Batch &b, OpList &L, const TileCtx &c) {
+    if (b.reg.empty()) return;
+    const int n_ent = 1 << c.R;
+    std::vector<cd> table(n_ent, 1.0);
+    bool any = false;
+    for (size_t i = 0; i < b.reg.size(); i++) {
+        if (b.r1[i] == cd(1.0)) continue;
+        any = true;
+        for (int j = 0; j < n_ent; j++)
+            if ((j >> b.reg[i]) & 1) table[j] *= b.r1[i];
+    }
+    if (any) emit_table(L, c, table);
+    dq_op z;
+    memset(&z, 0, sizeof z);
+    for (size_t i = 0; i < b.reg.size(); i++) {
+        const double m[4] = {b.M[i][0], b.M[i][1], b.M[i][2], b.M[i][3]};
+        L.add(DQ_OP_RMAT, (uint32_t)b.reg[i], z, m, 4);
+    }
+    b = Batch();
+}
+
+// Apply the pending diagonal of qubit q now.
+void apply_pending(Fuse &F, int q, OpList &L, const TileCtx &c) {
+    if (F.trivial(q)) return;
+    const int r = reg_of(c, q);
+    if (r >= 0) {
+        std::vector<cd> table(1 << c.R);
+        for (int j = 0; j < (1 << c.R); j++) table[j] = ((j >> r) & 1) ? F.p1[q] : F.p0[q];
+        emit_table(L, c, table);
+    } else {
+        F.g *= F.p0[q];
+        const cd ph = F.p1[q] / F.p0[q];
+        const double m[2] = {ph.real(), ph.imag()};
+        const uint64_t qm = 1ull << q;
+        L.add(DQ_OP_PHASE, 0, cond_of(c, false, qm, qm), m, 2);
+    }
+    F.p0[q] = F.p1[q] = 1.0;
+}
+
+// Apply everything pending (end of a flush): register qubits in one table,
+// the others as phase ops, the global factor last.
+void apply_all_pending(Fuse &F, OpList &L, const TileCtx &c, int n) {
+    const int n_ent = 1 << c.R;
+    std::vector<cd> table(n_ent, 1.0);
+    bool any = false;
+    for (int q = 0; q < n; q++) {
+        if (F.trivial(q)) continue;
+        const int r = reg_of(c, q);
+        if (r < 0) {
+            apply_pending(F, q, L, c);
+            continue;
+        }
+        any = true;
+        for (int j = 0; j < n_ent; j++) table[j] *= ((j >> r) & 1) ? F.p1[q] : F.p0[q];
+        F.p0[q] = F.p1[q] = 1.0;
+    }
+    if (F.g != cd(1.0)) {
+        any = true;
+        for (auto &t : table) t *= F.g;
+        F.g = 1.0;
+    }
+    if (any) emit_table(L, c, table);
+}
+
+// U = diag(l0, l1) * M * diag(1, r1) with M real.  Returns false when the
+// factorisation would drop more than rounding (a matrix that is unitary only
+// to the validation tolerance); the caller then keeps the complex matrix.
+bool real_factor(const cd U[4], cd &l0, cd &l1, cd &r1, double M[4]) {
+    auto unit = [](cd z) { const double a = std::abs(z); return a > 0 ? z / a : cd(1.0); };
+    const double tiny = 1e-300;
+    if (std::abs(U[1]) <= tiny && std::abs(U[2]) <= tiny) {
+        l0 = unit(U[0]); l1 = unit(U[3]); r1 = 1.0;
+    } else if (std::abs(U[0]) <= tiny && std::abs(U[3]) <= tiny) {
+        l0 = unit(U[1]); l1 = unit(U[2]); r1 = 1.0;
+    } else {
+        l0 = unit(U[0]); l1 = unit(U[2]); r1 = unit(U[1]) / l0;
+    }
+    const cd m[4] = {std::conj(l0) * U[0], std::conj(l0) * U[1] * std::conj(r1), std::conj(l1) * U[2],
+                     std::conj(l1) * U[3] * std::conj(r1)};
+    for (int i = 0; i < 4; i++) {
+        if (std::abs(m[i].imag()) > 4e-16 * (1.0 + std::abs(m[i]))) return false;
+        M[i] = m[i].real();
+    }
+    return true;
+}
+
+void emit_general(OpList &L, const TileCtx &c, bool small, uint32_t kind, uint32_t tbit, uint64_t cm, const double *m) {
+    L.add(kind, tbit, cond_of(c, small, cm, cm), m, 8);
+}
+
+// Diagonal gates with two or more qubits (or a control): emitted as they come.
+void emit_diagonal(const Gate &g, OpList &L, Fuse &F, const TileCtx &c, bool small) {
+    const uint64_t cm = g.cmask;
     switch (g.type) {
-        case OP_ONE: return false;
-        case OP_PAULI_Z: o.kind = DQ_OP_NEG; ctrl(g.cmask); return true;
-        case OP_T_GATE: case OP_S_GATE: case OP_T_GATE_CONJ_NEG: case OP_T_GATE_NEG:
-        case OP_S_GATE_CONJ: case OP_T_GATE_CONJ: {
-            // e^{i k pi/4}, QuEST_gpu_kernel.cu:17-64
+        case OP_ONE: return;
+        case OP_PAULI_Z: L.add(DQ_OP_NEG, 0, cond_of(c, small, cm, cm), nullptr, 0); return;
+        case OP_PHASE_SHIFT: {
+            const double p[2] = {std::cos(g.op[0]), std::sin(g.op[0])};
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm, cm), p, 2);
+            return;
+        }
+        case OP_ROTATE_Z: {  // controlled: diag(e^{-ia}, e^{ia}) on the target where the controls are 1
+            const double a = g.op[0];
+            const uint64_t t = g.tmask;
+            const double p0[2] = {std::cos(a), -std::sin(a)}, p1[2] = {std::cos(a), std::sin(a)};
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm), p0, 2);
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm | t), p1, 2);
+            return;
+        }
+        case OP_COLLAPSE: {  // keep the outcome branch scaled by 1/sqrt(P), zero the other
+            const int outcome = (int)g.op[0];
+            const uint64_t t = g.tmask;
+            const double zero[2] = {0, 0};
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm | (outcome ? 0 : t)), zero, 2);
+            if (cm == 0 && F.on) {
+                F.g *= g.op[1];
+            } else {
+                const double sc[2] = {g.op[1], 0};
+                L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm | (outcome ? t : 0)), sc, 2);
+            }
+            return;
+        }
+        default: {  // phase gates e^{i k pi/4} where the mask qubits are all 1 (QuEST_gpu_kernel.cu:17-64)
             static const double pr[8] = {1, kRsqrt2, 0, -kRsqrt2, -1, -kRsqrt2, 0, kRsqrt2};
             static const double pi[8] = {0, kRsqrt2, 1, kRsqrt2, 0, -kRsqrt2, -1, -kRsqrt2};
-            o.kind = DQ_OP_PHASE;
-            put(o, 0, pr[g.type], prec);
-            put(o, 1, pi[g.type], prec);
-            ctrl(g.cmask);
-            return true;
-        }
-        case OP_PHASE_SHIFT:
-            o.kind = DQ_OP_PHASE;
-            put(o, 0, std::cos(g.op[0]), prec);
-            put(o, 1, std::sin(g.op[0]), prec);
-            ctrl(g.cmask);
-            return true;
-        case OP_ROTATE_Z:  // diag(e^{-i a}, e^{+i a}) on the target bit, a = half angle
-            o.kind = DQ_OP_DIAG2;
-            put(o, 0, std::cos(g.op[0]), prec);
-            put(o, 1, -std::sin(g.op[0]), prec);
-            put(o, 2, std::cos(g.op[0]), prec);
-            put(o, 3, std::sin(g.op[0]), prec);
-            ctrl(g.cmask);
-            sel(g.tmask);
-            return true;
-        case OP_COLLAPSE: {
-            const int outcome = (int)g.op[0];
-            o.kind = DQ_OP_DIAG2;
-            put(o, 0, outcome == 0 ? g.op[1] : 0.0, prec);
-            put(o, 1, 0.0, prec);
-            put(o, 2, outcome == 1 ? g.op[1] : 0.0, prec);
-            put(o, 3, 0.0, prec);
-            ctrl(g.cmask);
-            sel(g.tmask);
-            return true;
-        }
-        case OP_PAULI_X:
-            o.kind = DQ_OP_SWAP;
-            o.tbit = target(g.tmask);
-            ctrl(g.cmask);
-            return true;
-        default: {
-            double m[8];
-            gate_matrix(g, m);
-            o.kind = DQ_OP_MAT;
-            o.tbit = target(g.tmask);
-            for (int i = 0; i < 8; i++) put(o, i, m[i], prec);
-            ctrl(g.cmask);
-            return true;
+            const double p[2] = {pr[g.type], pi[g.type]};
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm, cm), p, 2);
+            return;
         }
     }
+}
+
+// Single-qubit diagonal without control?  Returns its qubit and factors.
+bool single_diag(const Gate &g, int &q, cd &d0, cd &d1) {
+    if (g.type == OP_ROTATE_Z) {
+        if (g.cmask) return false;
+        q = ctz64(g.tmask);
+        d0 = std::polar(1.0, -g.op[0]);
+        d1 = std::polar(1.0, g.op[0]);
+        return true;
+    }
+    if (g.type > OP_PHASE_SHIFT || popc64(g.cmask) != 1) return false;
+    q = ctz64(g.cmask);
+    d0 = 1.0;
+    if (g.type == OP_PHASE_SHIFT) d1 = std::polar(1.0, g.op[0]);
+    else d1 = std::polar(1.0, std::atan(1.0) * g.type);
+    if (g.type == OP_PAULI_Z) d1 = -1.0;
+    if (g.type == OP_S_GATE) d1 = cd(0, 1);
+    if (g.type == OP_S_GATE_CONJ) d1 = cd(0, -1);
+    return true;
+}
+
+// Ops of one phase.  With fusion on, uncontrolled single-qubit gates become
+// real rotations plus pending diagonals.
+void pack_phase(const std::vector<Gate> &gates, Fuse &F, OpList &L, const TileCtx &c, bool small) {
+    Batch b;
+    for (const Gate &g : gates) {
+        const uint64_t touched = g.tmask | g.cmask;
+        if (gate_is_diagonal(g.type)) {
+            int q;
+            cd d0, d1;
+            if (F.on && single_diag(g, q, d0, d1)) {
+                F.p0[q] *= d0;
+                F.p1[q] *= d1;
+                continue;
+            }
+            if (b.qmask & touched) flush_batch(b, L, c);
+            emit_diagonal(g, L, F, c, small);
+            continue;
+        }
+        const int t = ctz64(g.tmask);
+        const uint32_t tb = small ? (uint32_t)t : (uint32_t)reg_of(c, t);
+        if (!F.on) {
+            double m[8];
+            gate_matrix(g, m);
+            emit_general(L, c, small, g.type == OP_PAULI_X ? DQ_OP_SWAP : DQ_OP_MAT, tb, g.cmask, m);
+            continue;
+        }
+        if (b.qmask & touched) flush_batch(b, L, c);
+        if (g.cmask) {
+            apply_pending(F, t, L, c);
+            double m[8];
+            gate_matrix(g, m);
+            emit_general(L, c, small, g.type == OP_PAULI_X ? DQ_OP_SWAP : DQ_OP_MAT, tb, g.cmask, m);
+            continue;
+        }
+        if (g.type == OP_PAULI_X) {  // X diag(p0, p1) = diag(p1, p0) X
+            std::swap(F.p0[t], F.p1[t]);
+            dq_op z;
+            memset(&z, 0, sizeof z);
+            L.add(DQ_OP_SWAP, tb, z, nullptr, 0);
+            continue;
+        }
+        double m[8];
+        gate_matrix(g, m);
+        const cd U[4] = {cd(m[0], m[1]) * F.p0[t], cd(m[2], m[3]) * F.p1[t], cd(m[4], m[5]) * F.p0[t],
+                         cd(m[6], m[7]) * F.p1[t]};
+        cd l0, l1, r1;
+        double M[4];
+        if (real_factor(U, l0, l1, r1, M)) {
+            b.reg.push_back((int)tb);
+            b.M.push_back({M[0], M[1], M[2], M[3]});
+            b.r1.push_back(r1);
+            b.qmask |= 1ull << t;
+            F.p0[t] = l0;
+            F.p1[t] = l1;
+        } else {
+            const double mu[8] = {U[0].real(), U[0].imag(), U[1].real(), U[1].imag(),
+                                  U[2].real(), U[2].imag(), U[3].real(), U[3].imag()};
+            F.p0[t] = F.p1[t] = 1.0;
+            emit_general(L, c, small, DQ_OP_MAT, tb, 0, mu);
+        }
+    }
+    flush_batch(b, L, c);
 }
 
 Layout standard_layout(int T, int R, int W) {
@@ -216,7 +442,7 @@ void set_end_masks(dq_block &blk, const Layout &first, const Layout &last, const
 }
 
 void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, const TuneParams &t, int prec, int init_kind,
-                uint64_t init_index, double init_value, Plan &P) {
+                uint64_t init_index, double init_value, Fuse &F, bool final, Plan &P) {
     const int T = 6 + t.R + t.W, R = t.R, W = t.W;
     // tile qubits: the block's targets (always containing 0..5), padded with
     // the lowest other qubits (src/cpp/QuEST_gpu_kernel.cu:505-510 pads low to high)
@@ -273,14 +499,14 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, const TunePa
     blk.phase_begin = (uint32_t)P.phases.size();
     blk.phase_count = (uint32_t)lays.size();
     set_end_masks(blk, lays.front(), lays.back(), tile_q, tile_q, R, W);
+    OpList L;
     for (size_t p = 0; p < lays.size(); p++) {
         dq_phase ph = phase_of(lays[p], R, W);
         ph.op_begin = (uint32_t)P.ops.size();
         c.roles = roles_of(lays[p], R, W);
-        for (const auto &g : *opsets[p]) {
-            dq_op o;
-            if (pack_gate(g, c, false, o)) P.ops.push_back(o);
-        }
+        pack_phase(*opsets[p], F, L, c, false);
+        if (final && p + 1 == lays.size()) apply_all_pending(F, L, c, n);
+        L.flush_to(P.ops, prec);
         ph.op_count = (uint32_t)P.ops.size() - ph.op_begin;
         P.phases.push_back(ph);
     }
@@ -320,10 +546,10 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         dq_phase ph;
         memset(&ph, 0, sizeof ph);
         ph.op_begin = 0;
-        for (const auto &g : gl) {
-            dq_op o;
-            if (pack_gate(g, c, true, o)) P.ops.push_back(o);
-        }
+        Fuse F(0);
+        OpList L;
+        pack_phase(gl, F, L, c, true);
+        L.flush_to(P.ops, prec);
         ph.op_count = (uint32_t)P.ops.size();
         P.phases.push_back(ph);
         P.blocks.push_back(blk);
@@ -336,17 +562,24 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
     const TuneParams t = fit_shape(tp, n);
     Dag d(n, true);
     for (const auto &g : gates) d.add(g);
-    bool first = true;
+    Fuse F(n);
+    F.on = tp.fuse_diagonals;
+    std::vector<std::vector<Gate>> bgs;
+    std::vector<uint64_t> tms;
     while (!d.empty()) {
         std::vector<Gate> bg;
         uint64_t tm = 0x3Full;
         d = d.select(bg, tm, 6 + t.k, (size_t)t.max_ops);
-        emit_block(bg, tm, n, t, prec, first ? init_kind : DQ_INIT_LOAD, init_index, init_value, P);
-        first = false;
+        bgs.push_back(std::move(bg));
+        tms.push_back(tm);
     }
-    if (first && init_kind != DQ_INIT_LOAD) {
-        emit_block(std::vector<Gate>(), 0x3Full, n, t, prec, init_kind, init_index, init_value, P);
+    if (bgs.empty() && init_kind != DQ_INIT_LOAD) {
+        bgs.emplace_back();
+        tms.push_back(0x3Full);
     }
+    for (size_t i = 0; i < bgs.size(); i++)
+        emit_block(bgs[i], tms[i], n, t, prec, i == 0 ? init_kind : DQ_INIT_LOAD, init_index, init_value, F,
+                   i + 1 == bgs.size(), P);
     P.st.ops = (int)P.ops.size();
 }
 

@@ -29,6 +29,7 @@ struct TuneParams {
     int R = 4;
     int W = 2;
     int max_ops = 4096;
+    bool fuse_diagonals = true;  // real rotations + deferred diagonals (see planner.cpp)
 };
 
 struct PlanStats {

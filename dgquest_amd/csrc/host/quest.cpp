@@ -189,6 +189,7 @@ TuneParams env_tune(int prec) {
         int k, R, W, mo;
         if (sscanf(e, "%d,%d,%d,%d", &k, &R, &W, &mo) == 4) { t.k = k; t.R = R; t.W = W; t.max_ops = mo; }
     }
+    if (const char *e = getenv("DGQ_FUSE")) t.fuse_diagonals = atoi(e) != 0;
     return t;
 }
 
@@ -244,14 +245,20 @@ static void run_plan(QuregState *s, const Plan &P) {
 void flush(QuregState *s) {
     if (s->pending.empty() && s->init_kind == DQ_INIT_LOAD) return;
     const auto t0 = std::chrono::steady_clock::now();
+    const unsigned long long state_bytes = (unsigned long long)(s->prec == DQ_F64 ? 16 : 8) << s->n;
     if (s->pending.empty()) {
         ENG(dq_init_state(s->d_state, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->stream));
         s->last_stats = PlanStats();
+        s->last_bytes = state_bytes;
+        s->last_launches = 1;
     } else {
         Plan P;
         plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P);
         run_plan(s, P);
         s->last_stats = P.st;
+        s->last_launches = (int)P.blocks.size();
+        s->last_bytes = 0;
+        for (const auto &b : P.blocks) s->last_bytes += (b.init_kind == DQ_INIT_LOAD ? 2 : 1) * state_bytes;
     }
     s->pending.clear();
     s->init_kind = DQ_INIT_LOAD;
@@ -781,6 +788,14 @@ int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops) {
     return 0;
 }
 
+int dgq_set_fusion(Qureg qureg, int on) {
+    QuregState *s = st(qureg);
+    if (!s) return -1;
+    flush(s);
+    s->tune.fuse_diagonals = on != 0;
+    return 0;
+}
+
 int dgq_last_plan_stats(Qureg qureg, int stats[8], double *flush_ms) {
     QuregState *s = st(qureg);
     if (!s || !stats) return -1;
@@ -794,6 +809,19 @@ int dgq_last_plan_stats(Qureg qureg, int stats[8], double *flush_ms) {
     stats[6] = t.R;
     stats[7] = t.W;
     if (flush_ms) *flush_ms = s->last_flush_ms;
+    return 0;
+}
+
+void *dgq_stream(Qureg qureg) {
+    QuregState *s = st(qureg);
+    return s ? s->stream : nullptr;
+}
+
+int dgq_last_flush_bytes(Qureg qureg, unsigned long long *bytes, int *launches) {
+    QuregState *s = st(qureg);
+    if (!s) return -1;
+    if (bytes) *bytes = s->last_bytes;
+    if (launches) *launches = s->last_launches;
     return 0;
 }
 

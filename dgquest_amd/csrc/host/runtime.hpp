@@ -36,6 +36,8 @@ struct QuregState {
     std::vector<double> p0;           // per physical qubit, total at [n]
     dgq::TuneParams tune;
     dgq::PlanStats last_stats;
+    unsigned long long last_bytes = 0;  // algorithmic HBM bytes of the last flush
+    int last_launches = 0;
     double last_flush_ms = 0.0;
 };
 

@@ -129,6 +129,12 @@ class QuEST:
         L.dgq_last_plan_stats.argtypes = [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.dgq_set_tuning.restype = C.c_int
         L.dgq_set_tuning.argtypes = [Qureg, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.dgq_set_fusion.restype = C.c_int
+        L.dgq_set_fusion.argtypes = [Qureg, C.c_int]
+        L.dgq_stream.restype = C.c_void_p
+        L.dgq_stream.argtypes = [Qureg]
+        L.dgq_last_flush_bytes.restype = C.c_int
+        L.dgq_last_flush_bytes.argtypes = [Qureg, C.POINTER(C.c_ulonglong), C.POINTER(C.c_int)]
         L.dgq_sync.restype = C.c_int
         L.dgq_sync.argtypes = [Qureg]
         L.dgq_flush.restype = C.c_int
@@ -197,6 +203,13 @@ class QuEST:
         if rc != 0:
             raise RuntimeError("dgq_read_state failed")
         return out.view(np.complex128)
+
+    def flush_bytes(self, q: Qureg):
+        """(algorithmic HBM bytes, sweep launches) of the last flush."""
+        b = C.c_ulonglong()
+        l = C.c_int()
+        self.lib.dgq_last_flush_bytes(q, C.byref(b), C.byref(l))
+        return b.value, l.value
 
     def plan_stats(self, q: Qureg):
         """Statistics of the last flush: sweeps, phases, exchanges, ops, gates_in, k, R, W + flush ms."""

@@ -72,22 +72,30 @@ typedef struct dq_phase {
 #define DQ_OP_MAT 1   /* 2x2 complex matrix on register bit tbit: m = u00,u01,u10,u11 (re,im) */
 #define DQ_OP_NEG 2   /* amplitude *= -1 */
 #define DQ_OP_PHASE 3 /* amplitude *= (m[0] + i m[1]) */
-#define DQ_OP_DIAG2 4 /* amplitude *= sel ? (m[2] + i m[3]) : (m[0] + i m[1]); sel = selector bit */
 #define DQ_OP_SWAP 5  /* X on register bit tbit (exchange the pair) */
+#define DQ_OP_RMAT 6  /* real 2x2 matrix [[m0, m1], [m2, m3]] on register bit tbit */
+#define DQ_OP_DIAGR 7 /* amplitude of register index j *= table[j]; the table (2^R complex)
+                       * follows in the next tbit ops, 4 complex (re, im) per op in m[] */
 
-/* Controls: an op acts on an amplitude only if all its control bits are 1:
- *   (reg_index & c_reg) == c_reg, (threadIdx.x & c_thr) == c_thr, and
- *   (tile_base & c_wg) == c_wg, where tile_base holds the non-tile qubits.
- * The selector of DQ_OP_DIAG2 is one bit in exactly one of s_reg/s_thr/s_wg. */
+/* Conditions: an op acts on an amplitude only if
+ *   (reg_index & c_reg) == v_reg, (threadIdx.x & c_thr) == v_thr and
+ *   (tile_base & c_wg) == v_wg,
+ * where reg_index is the amplitude's register index in its thread and
+ * tile_base holds the non-tile qubits of the workgroup.  Controls are bits
+ * required to be 1; a condition on a bit being 0 (projections) sets the
+ * mask bit and leaves the value bit 0.  For DQ_OP_MAT / DQ_OP_SWAP the
+ * condition is tested on the pair's lower index and must not involve tbit.
+ * m holds 8 values of the state precision (double for DQ_F64, float for
+ * DQ_F32, packed from the start of the array). */
 typedef struct dq_op {
     uint32_t kind;
     uint32_t tbit;
     uint32_t c_reg;
+    uint32_t v_reg;
     uint32_t c_thr;
+    uint32_t v_thr;
     uint64_t c_wg;
-    uint32_t s_reg;
-    uint32_t s_thr;
-    uint64_t s_wg;
+    uint64_t v_wg;
     double m[8];
 } dq_op;
 

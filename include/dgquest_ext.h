@@ -22,6 +22,11 @@ extern "C" {
  * clamped to what the register size and the built kernels allow. */
 int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops);
 
+/* Diagonal fusion on (1, default) or off (0): with it, uncontrolled
+ * single-qubit gates run as real 2x2 rotations and their diagonal parts are
+ * deferred and merged (fewer floating-point operations per gate). */
+int dgq_set_fusion(Qureg qureg, int on);
+
 /* stats[0..7] = sweeps, phases, LDS exchanges, device ops, recorded gates,
  * k, R, W of the last flush; *flush_ms = host time of that flush (planning
  * and launch, not waiting for the GPU). */
@@ -36,6 +41,15 @@ int dgq_sync(Qureg qureg);
 /* Flush, then copy amplitudes start..start+count-1 (logical order) into
  * out[2*count] as {re, im} doubles. */
 int dgq_read_state(Qureg qureg, unsigned long long start, unsigned long long count, double *out);
+
+/* The HIP stream (hipStream_t) this register's kernels run on, for event
+ * timing around its sweeps. */
+void *dgq_stream(Qureg qureg);
+
+/* Algorithmic HBM bytes of the last flush: every sweep reads and writes the
+ * state once, except a sweep that generates its input (init fused) which
+ * only writes; *launches = number of sweep launches. */
+int dgq_last_flush_bytes(Qureg qureg, unsigned long long *bytes, int *launches);
 
 /* Planning only, no GPU needed: plan `ngates` gate records on an n-qubit
  * state with the given shape and report stats[0..7] as above. */

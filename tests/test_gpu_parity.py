@@ -54,6 +54,23 @@ def test_random_circuits_all_gates(quest, env, oracle, n):
     quest.destroyQureg(q, env)
 
 
+@pytest.mark.parametrize("fuse", [0, 1])
+@pytest.mark.parametrize("n", [7, 12, 17])
+def test_diagonal_fusion_on_off(quest, env, oracle, n, fuse):
+    """Real-rotation fusion and the plain complex path give the oracle's state."""
+    rng = np.random.default_rng(300 + n)
+    circ = H.random_circuit(rng, n, 120)
+    q = quest.createQureg(n, env)
+    quest.dgq_set_fusion(q, fuse)
+    quest.apply_circuit(q, circ)
+    sv = quest.state_vector(q)
+    ref = run_oracle(oracle, n, circ)
+    err = np.abs(sv - ref).max()
+    print(f"n={n} fuse={fuse} err={err:.3e} stats={quest.plan_stats(q)}")
+    assert err <= TOL
+    quest.destroyQureg(q, env)
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 0, 64), (3, 3, 0, 64), (4, 4, 0, 8), (5, 3, 2, 4096), (6, 4, 2, 4096),
                                    (6, 5, 1, 4096), (7, 4, 3, 4096), (7, 5, 2, 4096), (2, 2, 1, 3), (4, 2, 2, 16),
                                    (6, 3, 3, 4096)])
