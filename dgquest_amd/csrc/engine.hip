@@ -72,253 +72,83 @@ __device__ __forceinline__ uint32_t thread_tile_addr(const dq_phase *__restrict_
     return t;
 }
 
+// The register tile of one thread: 2^R amplitudes, real and imaginary parts
+// as separate scalars.  Every op below updates these registers in place, so
+// the tile keeps one register assignment through the op loop.
+template <typename T, int R>
+struct Regs {
+    T re[1 << R];
+    T im[1 << R];
+};
+
 // Re-distribute the tile from the layout of phase A to that of phase B
 // through LDS (each thread writes its 2^R amplitudes, then reads 2^R).
 // SPLIT moves the real parts, then the imaginary parts, through an LDS array
 // of scalars: half the LDS footprint (more workgroups per CU), two more
-// barriers.
-template <int R, int W, bool SPLIT, typename V>
-__device__ __forceinline__ void exchange(V (&a)[1 << R], void *lds_raw, const dq_phase *__restrict__ A,
+// barriers.  Offsets are in bytes: the thread's part xor a uniform part per
+// register (the swizzle is linear), one vector xor per access.
+template <int R, int W, bool SPLIT, typename T>
+__device__ __forceinline__ void exchange(Regs<T, R> &a, void *lds_raw, const dq_phase *__restrict__ A,
                                          const dq_phase *__restrict__ B, bool lead_barrier, uint32_t tid) {
-    using T = typename std::remove_reference<decltype(a[0].x)>::type;
-    const uint32_t sa = swz(thread_tile_addr<W>(A, tid));
-    const uint32_t sb = swz(thread_tile_addr<W>(B, tid));
+    constexpr int NR = 1 << R;
+    constexpr uint32_t ES = SPLIT ? sizeof(T) : 2 * sizeof(T);
+    const uint32_t sa = swz(thread_tile_addr<W>(A, tid)) * ES;
+    const uint32_t sb = swz(thread_tile_addr<W>(B, tid)) * ES;
     uint32_t ra[R > 0 ? R : 1], rb[R > 0 ? R : 1];
 #pragma unroll
-    for (int r = 0; r < R; r++) { ra[r] = swz(A->reg_t[r]); rb[r] = swz(B->reg_t[r]); }
-    // LDS element index of register j: lane/wave part xor the uniform register part
-    auto off = [&](uint32_t s0, const uint32_t *rr, int j) {
-        uint32_t x = 0;
+    for (int r = 0; r < R; r++) { ra[r] = swz(A->reg_t[r]) * ES; rb[r] = swz(B->reg_t[r]) * ES; }
+    uint32_t oa[NR], ob[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        uint32_t xa = 0, xb = 0;
 #pragma unroll
         for (int r = 0; r < R; r++)
-            if ((j >> r) & 1) x ^= rr[r];
-        return s0 ^ x;
-    };
+            if ((j >> r) & 1) { xa ^= ra[r]; xb ^= rb[r]; }
+        oa[j] = sa ^ xa;
+        ob[j] = sb ^ xb;
+    }
+    unsigned char *lds = reinterpret_cast<unsigned char *>(lds_raw);
     if (!SPLIT) {
-        V *lds = reinterpret_cast<V *>(lds_raw);
         if (lead_barrier) __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) lds[off(sa, ra, j)] = a[j];
+        for (int j = 0; j < NR; j++) {
+            *reinterpret_cast<T *>(lds + oa[j]) = a.re[j];
+            *reinterpret_cast<T *>(lds + oa[j] + sizeof(T)) = a.im[j];
+        }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) a[j] = lds[off(sb, rb, j)];
+        for (int j = 0; j < NR; j++) {
+            a.re[j] = *reinterpret_cast<const T *>(lds + ob[j]);
+            a.im[j] = *reinterpret_cast<const T *>(lds + ob[j] + sizeof(T));
+        }
     } else {
-        T *lds = reinterpret_cast<T *>(lds_raw);
         if (lead_barrier) __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) lds[off(sa, ra, j)] = a[j].x;
-        __syncthreads();
-        T re[1 << R];
-#pragma unroll
-        for (int j = 0; j < (1 << R); j++) re[j] = lds[off(sb, rb, j)];
+        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + oa[j]) = a.re[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) lds[off(sa, ra, j)] = a[j].y;
+        for (int j = 0; j < NR; j++) a.re[j] = *reinterpret_cast<const T *>(lds + ob[j]);
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < (1 << R); j++) { a[j].y = lds[off(sb, rb, j)]; a[j].x = re[j]; }
-    }
-}
-
-// ---- op bodies ---------------------------------------------------------------
-// Every op works in place on the register tile a[].  Conditions on register
-// bits are uniform per register index (scalar branches); conditions on lane or
-// wave bits become per-lane coefficients chosen once per op.
-
-template <typename T> struct SignBits;
-template <> struct SignBits<double> {
-    __device__ static double flip(double x, uint32_t hi_mask) {
-        uint64_t b = __builtin_bit_cast(uint64_t, x);
-        b ^= (uint64_t)hi_mask << 32;
-        return __builtin_bit_cast(double, b);
-    }
-};
-template <> struct SignBits<float> {
-    __device__ static float flip(float x, uint32_t hi_mask) {
-        return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ hi_mask);
-    }
-};
-
-// 2x2 complex matrix on register bit B applied to every pair (no condition).
-template <int B, int R, typename T, typename V>
-__device__ __forceinline__ void op_mat(V (&a)[1 << R], T u0r, T u0i, T u1r, T u1i, T u2r, T u2i, T u3r, T u3i) {
+        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + oa[j]) = a.im[j];
+        __syncthreads();
 #pragma unroll
-    for (int j0 = 0; j0 < (1 << R); j0++) {
-        if (j0 & (1 << B)) continue;
-        const int j1 = j0 | (1 << B);
-        const V x = a[j0], y = a[j1];
-        V nx, ny;
-        nx.x = u0r * x.x - u0i * x.y + u1r * y.x - u1i * y.y;
-        nx.y = u0r * x.y + u0i * x.x + u1r * y.y + u1i * y.x;
-        ny.x = u2r * x.x - u2i * x.y + u3r * y.x - u3i * y.y;
-        ny.y = u2r * x.y + u2i * x.x + u3r * y.y + u3i * y.x;
-        a[j0] = nx;
-        a[j1] = ny;
+        for (int j = 0; j < NR; j++) a.im[j] = *reinterpret_cast<const T *>(lds + ob[j]);
     }
 }
 
-// Same, only on pairs whose lower register index satisfies the condition.
-template <int B, int R, typename T, typename V>
-__device__ __forceinline__ void op_mat_cond(V (&a)[1 << R], T u0r, T u0i, T u1r, T u1i, T u2r, T u2i, T u3r,
-                                            T u3i, uint32_t creg, uint32_t vreg) {
-#pragma unroll
-    for (int j0 = 0; j0 < (1 << R); j0++) {
-        if (j0 & (1 << B)) continue;
-        if (((uint32_t)j0 & creg) != vreg) continue;
-        const int j1 = j0 | (1 << B);
-        const V x = a[j0], y = a[j1];
-        V nx, ny;
-        nx.x = u0r * x.x - u0i * x.y + u1r * y.x - u1i * y.y;
-        nx.y = u0r * x.y + u0i * x.x + u1r * y.y + u1i * y.x;
-        ny.x = u2r * x.x - u2i * x.y + u3r * y.x - u3i * y.y;
-        ny.y = u2r * x.y + u2i * x.x + u3r * y.y + u3i * y.x;
-        a[j0] = nx;
-        a[j1] = ny;
-    }
-}
-
-template <int B, int R, typename V>
-__device__ __forceinline__ void op_swap(V (&a)[1 << R], uint32_t creg, uint32_t vreg, bool tok) {
-#pragma unroll
-    for (int j0 = 0; j0 < (1 << R); j0++) {
-        if (j0 & (1 << B)) continue;
-        if (((uint32_t)j0 & creg) != vreg) continue;
-        const int j1 = j0 | (1 << B);
-        const V x = a[j0], y = a[j1];
-        a[j0] = tok ? y : x;
-        a[j1] = tok ? x : y;
-    }
-}
-
-template <int R, typename T, typename V>
-__device__ __forceinline__ void dispatch_mat(V (&a)[1 << R], uint32_t tb, const T *__restrict__ u, uint32_t creg,
-                                             uint32_t vreg, bool lane_cond, bool tok) {
-    T c0r = u[0], c0i = u[1], c1r = u[2], c1i = u[3], c2r = u[4], c2i = u[5], c3r = u[6], c3i = u[7];
-    if (lane_cond) {  // lanes failing the condition apply the identity
-        c0r = tok ? c0r : T(1); c0i = tok ? c0i : T(0); c1r = tok ? c1r : T(0); c1i = tok ? c1i : T(0);
-        c2r = tok ? c2r : T(0); c2i = tok ? c2i : T(0); c3r = tok ? c3r : T(1); c3i = tok ? c3i : T(0);
-    }
-    if (creg == 0) {
-        switch (tb) {
-            case 0: if constexpr (R > 0) op_mat<0, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
-            case 1: if constexpr (R > 1) op_mat<1, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
-            case 2: if constexpr (R > 2) op_mat<2, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
-            case 3: if constexpr (R > 3) op_mat<3, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
-            case 4: if constexpr (R > 4) op_mat<4, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i); break;
-            default: break;
-        }
-    } else {
-        switch (tb) {
-            case 0: if constexpr (R > 0) op_mat_cond<0, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
-            case 1: if constexpr (R > 1) op_mat_cond<1, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
-            case 2: if constexpr (R > 2) op_mat_cond<2, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
-            case 3: if constexpr (R > 3) op_mat_cond<3, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
-            case 4: if constexpr (R > 4) op_mat_cond<4, R>(a, c0r, c0i, c1r, c1i, c2r, c2i, c3r, c3i, creg, vreg); break;
-            default: break;
-        }
-    }
-}
-
-// real 2x2 matrix on register bit B: 8 FP64 (or FP32) operations per pair
-template <int B, int R, typename T, typename V>
-__device__ __forceinline__ void op_rmat(V (&a)[1 << R], T m0, T m1, T m2, T m3) {
-#pragma unroll
-    for (int j0 = 0; j0 < (1 << R); j0++) {
-        if (j0 & (1 << B)) continue;
-        const int j1 = j0 | (1 << B);
-        const V x = a[j0], y = a[j1];
-        V nx, ny;
-        nx.x = m0 * x.x + m1 * y.x;
-        nx.y = m0 * x.y + m1 * y.y;
-        ny.x = m2 * x.x + m3 * y.x;
-        ny.y = m2 * x.y + m3 * y.y;
-        a[j0] = nx;
-        a[j1] = ny;
-    }
-}
-
-template <int R, typename T, typename V>
-__device__ __forceinline__ void dispatch_rmat(V (&a)[1 << R], uint32_t tb, const T *__restrict__ m) {
-    const T m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
-    switch (tb) {
-        case 0: if constexpr (R > 0) op_rmat<0, R>(a, m0, m1, m2, m3); break;
-        case 1: if constexpr (R > 1) op_rmat<1, R>(a, m0, m1, m2, m3); break;
-        case 2: if constexpr (R > 2) op_rmat<2, R>(a, m0, m1, m2, m3); break;
-        case 3: if constexpr (R > 3) op_rmat<3, R>(a, m0, m1, m2, m3); break;
-        case 4: if constexpr (R > 4) op_rmat<4, R>(a, m0, m1, m2, m3); break;
-        default: break;
-    }
-}
-
-template <int R, typename V>
-__device__ __forceinline__ void dispatch_swap(V (&a)[1 << R], uint32_t tb, uint32_t creg, uint32_t vreg, bool tok) {
-    switch (tb) {
-        case 0: if constexpr (R > 0) op_swap<0, R>(a, creg, vreg, tok); break;
-        case 1: if constexpr (R > 1) op_swap<1, R>(a, creg, vreg, tok); break;
-        case 2: if constexpr (R > 2) op_swap<2, R>(a, creg, vreg, tok); break;
-        case 3: if constexpr (R > 3) op_swap<3, R>(a, creg, vreg, tok); break;
-        case 4: if constexpr (R > 4) op_swap<4, R>(a, creg, vreg, tok); break;
-        default: break;
-    }
-}
-
-template <int R, typename T, typename V>
-__device__ __forceinline__ void run_ops(V (&a)[1 << R], const dq_op *__restrict__ ops, uint32_t count,
-                                        uint64_t base, uint32_t tid) {
-    for (uint32_t k = 0; k < count; k++) {
-        const dq_op *__restrict__ op = ops + k;
-        if ((base & op->c_wg) != op->v_wg) continue;
-        const uint32_t kind = op->kind;
-        const uint32_t creg = op->c_reg, vreg = op->v_reg;
-        const uint32_t cthr = op->c_thr;
-        const bool tok = (tid & cthr) == op->v_thr;
-        const T *__restrict__ m = reinterpret_cast<const T *>(op->m);
-        if (kind == DQ_OP_MAT) {
-            dispatch_mat<R, T>(a, op->tbit, m, creg, vreg, cthr != 0, tok);
-        } else if (kind == DQ_OP_NEG) {
-            const uint32_t hm = (cthr == 0 || tok) ? (sizeof(T) == 8 ? 0x80000000u : 0x80000000u) : 0u;
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) {
-                if (((uint32_t)j & creg) != vreg) continue;
-                a[j].x = SignBits<T>::flip(a[j].x, hm);
-                a[j].y = SignBits<T>::flip(a[j].y, hm);
-            }
-        } else if (kind == DQ_OP_PHASE) {
-            T pr = m[0], pi = m[1];
-            if (cthr != 0) { pr = tok ? pr : T(1); pi = tok ? pi : T(0); }
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) {
-                if (((uint32_t)j & creg) != vreg) continue;
-                const V x = a[j];
-                a[j].x = pr * x.x - pi * x.y;
-                a[j].y = pr * x.y + pi * x.x;
-            }
-        } else if (kind == DQ_OP_SWAP) {
-            dispatch_swap<R>(a, op->tbit, creg, vreg, cthr == 0 || tok);
-        } else if (kind == DQ_OP_RMAT) {
-            dispatch_rmat<R, T>(a, op->tbit, m);
-        } else if (kind == DQ_OP_DIAGR) {
-            // per-register-index phases, uniform across the workgroup
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) {
-                const T *__restrict__ e = reinterpret_cast<const T *>(op[1 + (j >> 2)].m) + 2 * (j & 3);
-                const T pr = e[0], pi = e[1];
-                const V x = a[j];
-                a[j].x = pr * x.x - pi * x.y;
-                a[j].y = pr * x.y + pi * x.x;
-            }
-            k += op->tbit;
-        }
-    }
-}
+// ---- op interpreter --------------------------------------------------------
+// The op list of a phase runs in gfx950 assembly generated by gen_ops_asm.py
+// (format of the device program and register use documented there).
+#include "ops_asm.inc"
 
 // Occupancy target (waves per SIMD) of a sweep instantiation: the register
-// tile takes 2^R amplitudes x (2 * sizeof(T) / 4) VGPRs; allow about as many
-// again (at least 48) for addresses, matrix entries and temporaries.
+// tile takes 2^R amplitudes x (2 * sizeof(T) / 4) VGPRs; allow 40 more for
+// addresses, coefficients and temporaries.
 template <typename T, int R>
 constexpr int sweep_waves_per_eu() {
     constexpr int amp = (1 << R) * (int)(2 * sizeof(T) / 4);
-    constexpr int budget = amp + (amp > 48 ? amp : 48) + 24;
+    constexpr int budget = amp + 40;
     constexpr int w = 512 / budget;
     return w > 8 ? 8 : (w < 1 ? 1 : w);
 }
@@ -326,7 +156,8 @@ constexpr int sweep_waves_per_eu() {
 template <typename T, int R, int W, bool SPLIT>
 __global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R>()))
 sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
-             const dq_phase *__restrict__ phases, const dq_op *__restrict__ ops) {
+             const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
+             const uint32_t *__restrict__ pprog) {
     using V = typename V2<T>::type;
     using GV = __attribute__((address_space(1))) V;
     constexpr int NR = 1 << R;
@@ -352,16 +183,17 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
 
     const dq_phase *__restrict__ ph = phases + blk->phase_begin;
     const uint32_t nph = blk->phase_count;
-    V a[NR];
+    Regs<T, R> a;
 
+    // The wave index is uniform across a wavefront: with it in a scalar
+    // register the whole address except the lane part is scalar, and every
+    // load and store is "scalar base + lane * sizeof(V)" with no vector math.
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t lane = tid & 63u;
     {   // load: lane l = qubit l, register / wave bits = ld_reg / ld_wave.
-        // Address = uniform part (tile base | register bits) + per-lane part.
-        uint32_t toff = tid & 63u;
+        uint64_t wb = 0;
 #pragma unroll
-        for (int w = 0; w < W; w++) toff |= ((tid >> (6 + w)) & 1u) ? (uint32_t)blk->ld_wave[w] : 0u;
-        uint64_t whi = 0;  // wave bits above bit 31 go to the uniform part per wave
-#pragma unroll
-        for (int w = 0; w < W; w++) whi |= ((tid >> (6 + w)) & 1u) ? (blk->ld_wave[w] & ~0xFFFFFFFFull) : 0ull;
+        for (int w = 0; w < W; w++) wb |= ((wv >> w) & 1u) ? blk->ld_wave[w] : 0ull;
         uint64_t qo[R > 0 ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; r++) qo[r] = blk->ld_reg[r];
@@ -369,192 +201,58 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
         if (ik == DQ_INIT_LOAD) {
 #pragma unroll
             for (int j = 0; j < NR; j++) {
-                uint64_t u = base;
+                uint64_t u = base | wb;
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if ((j >> r) & 1) u |= qo[r];
-                const GV *p = st + (u | whi);
-                a[j] = __builtin_nontemporal_load(p + toff);
+                const GV *p = st + u;
+                const V v = __builtin_nontemporal_load(p + lane);
+                a.re[j] = v.x;
+                a.im[j] = v.y;
             }
         } else if (ik == DQ_INIT_CLASSICAL) {
             const uint64_t want = blk->init_index;
 #pragma unroll
             for (int j = 0; j < NR; j++) {
-                uint64_t u = base | whi | toff;
+                uint64_t u = base | wb;
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if ((j >> r) & 1) u |= qo[r];
-                a[j].x = (u == want) ? T(1) : T(0);
-                a[j].y = T(0);
+                a.re[j] = ((u | lane) == want) ? T(1) : T(0);
+                a.im[j] = T(0);
             }
         } else {
             const T v = (T)blk->init_value;
 #pragma unroll
-            for (int j = 0; j < NR; j++) { a[j].x = v; a[j].y = T(0); }
+            for (int j = 0; j < NR; j++) { a.re[j] = v; a.im[j] = T(0); }
         }
     }
 
+    const uint32_t *__restrict__ pp = pprog + 2 * blk->phase_begin;
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W, SPLIT>(a, (void *)lds_raw, ph + (p - 1), ph + p, p > 1, tid);
-        run_ops<R, T>(a, ops + ph[p].op_begin, ph[p].op_count, base, tid);
+        if (p > 0) exchange<R, W, SPLIT, T>(a, (void *)lds_raw, ph + (p - 1), ph + p, p > 1, tid);
+        AsmOps<T, R>::run(a, prog + pp[2 * p], pp[2 * p + 1], base, tid);
     }
 
     {   // store: lane l = qubit l, register / wave bits = st_reg / st_wave
-        uint32_t toff = tid & 63u;
+        uint64_t wb = 0;
 #pragma unroll
-        for (int w = 0; w < W; w++) toff |= ((tid >> (6 + w)) & 1u) ? (uint32_t)blk->st_wave[w] : 0u;
-        uint64_t whi = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) whi |= ((tid >> (6 + w)) & 1u) ? (blk->st_wave[w] & ~0xFFFFFFFFull) : 0ull;
+        for (int w = 0; w < W; w++) wb |= ((wv >> w) & 1u) ? blk->st_wave[w] : 0ull;
         uint64_t qo[R > 0 ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; r++) qo[r] = blk->st_reg[r];
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            uint64_t u = base;
+            uint64_t u = base | wb;
 #pragma unroll
             for (int r = 0; r < R; r++)
                 if ((j >> r) & 1) u |= qo[r];
-            GV *p = st + (u | whi);
-            __builtin_nontemporal_store(a[j], p + toff);
+            GV *p = st + u;
+            V v;
+            v.x = a.re[j];
+            v.y = a.im[j];
+            __builtin_nontemporal_store(v, p + lane);
         }
-    }
-}
-
-// ---- persistent, software-pipelined sweep -----------------------------------
-// One job (one state) per launch; the grid is sized to what stays resident and
-// each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...  The
-// loads of the next tile are issued into a second register set before the
-// current tile is computed and stored, so every wave keeps HBM busy while its
-// VALU and LDS work runs.
-template <typename T, int R, int W>
-struct TileIO {
-    using V = typename V2<T>::type;
-    using GV = __attribute__((address_space(1))) V;
-
-    __device__ static uint64_t tile_base(const dq_block *__restrict__ blk, uint64_t t) {
-        uint64_t base = 0;
-        uint64_t msk = blk->nontile_mask;
-        while (msk) {
-            const int p = __builtin_ctzll(msk);
-            base |= (t & 1ull) << p;
-            t >>= 1;
-            msk &= msk - 1;
-        }
-        return base;
-    }
-
-    __device__ static void load(V (&a)[1 << R], GV *__restrict__ st, const dq_block *__restrict__ blk,
-                                uint64_t base, uint32_t tid) {
-        uint32_t toff = tid & 63u;
-        uint64_t whi = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            const bool on = (tid >> (6 + w)) & 1u;
-            toff |= on ? (uint32_t)blk->ld_wave[w] : 0u;
-            whi |= on ? (blk->ld_wave[w] & ~0xFFFFFFFFull) : 0ull;
-        }
-        const uint32_t ik = blk->init_kind;
-        if (ik == DQ_INIT_LOAD) {
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) {
-                uint64_t u = base;
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    if ((j >> r) & 1) u |= blk->ld_reg[r];
-                a[j] = __builtin_nontemporal_load(st + (u | whi) + toff);
-            }
-        } else if (ik == DQ_INIT_CLASSICAL) {
-            const uint64_t want = blk->init_index;
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) {
-                uint64_t u = base | whi | toff;
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    if ((j >> r) & 1) u |= blk->ld_reg[r];
-                a[j].x = (u == want) ? T(1) : T(0);
-                a[j].y = T(0);
-            }
-        } else {
-            const T v = (T)blk->init_value;
-#pragma unroll
-            for (int j = 0; j < (1 << R); j++) { a[j].x = v; a[j].y = T(0); }
-        }
-    }
-
-    __device__ static void store(V (&a)[1 << R], GV *__restrict__ st, const dq_block *__restrict__ blk,
-                                 uint64_t base, uint32_t tid) {
-        uint32_t toff = tid & 63u;
-        uint64_t whi = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            const bool on = (tid >> (6 + w)) & 1u;
-            toff |= on ? (uint32_t)blk->st_wave[w] : 0u;
-            whi |= on ? (blk->st_wave[w] & ~0xFFFFFFFFull) : 0ull;
-        }
-#pragma unroll
-        for (int j = 0; j < (1 << R); j++) {
-            uint64_t u = base;
-#pragma unroll
-            for (int r = 0; r < R; r++)
-                if ((j >> r) & 1) u |= blk->st_reg[r];
-            __builtin_nontemporal_store(a[j], st + (u | whi) + toff);
-        }
-    }
-};
-
-template <typename T, int R, int W, bool SPLIT>
-__device__ __forceinline__ void compute_tile(typename V2<T>::type (&a)[1 << R], void *lds,
-                                             const dq_phase *__restrict__ ph, uint32_t nph,
-                                             const dq_op *__restrict__ ops, uint64_t base, uint32_t tid) {
-    for (uint32_t p = 0; p < nph; p++) {
-        // every exchange starts with a barrier: the previous tile's last
-        // exchange may still be read by other waves
-        if (p > 0) exchange<R, W, SPLIT>(a, lds, ph + (p - 1), ph + p, true, tid);
-        run_ops<R, T>(a, ops + ph[p].op_begin, ph[p].op_count, base, tid);
-    }
-}
-
-template <typename T, int R, int W, bool SPLIT>
-__global__ void __launch_bounds__(64 << W, 1)
-sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
-                  const dq_phase *__restrict__ phases, const dq_op *__restrict__ ops, uint64_t ntiles) {
-    using V = typename V2<T>::type;
-    using IO = TileIO<T, R, W>;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const dq_job job = jobs[0];
-    const dq_block *__restrict__ blk = blocks + job.block;
-    typename IO::GV *__restrict__ st = (typename IO::GV *)(job.state);
-    const uint32_t tid = threadIdx.x;
-    const dq_phase *__restrict__ ph = phases + blk->phase_begin;
-    const uint32_t nph = blk->phase_count;
-
-    V a[1 << R], b[1 << R];
-    uint64_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    uint64_t base_t = IO::tile_base(blk, t);
-    IO::load(a, st, blk, base_t, tid);
-    while (true) {
-        const uint64_t t2 = t + gridDim.x;
-        uint64_t base2 = 0;
-        if (t2 < ntiles) {
-            base2 = IO::tile_base(blk, t2);
-            IO::load(b, st, blk, base2, tid);
-        }
-        compute_tile<T, R, W, SPLIT>(a, (void *)lds_raw, ph, nph, ops, base_t, tid);
-        IO::store(a, st, blk, base_t, tid);
-        if (t2 >= ntiles) break;
-        const uint64_t t3 = t2 + gridDim.x;
-        uint64_t base3 = 0;
-        if (t3 < ntiles) {
-            base3 = IO::tile_base(blk, t3);
-            IO::load(a, st, blk, base3, tid);
-        }
-        compute_tile<T, R, W, SPLIT>(b, (void *)lds_raw, ph, nph, ops, base2, tid);
-        IO::store(b, st, blk, base2, tid);
-        if (t3 >= ntiles) break;
-        t = t3;
-        base_t = base3;
     }
 }
 
@@ -586,7 +284,7 @@ small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
             const T *__restrict__ m = reinterpret_cast<const T *>(op->m);
             const uint64_t c = op->c_wg, v = op->v_wg;
             __syncthreads();
-            if (op->kind == DQ_OP_MAT || op->kind == DQ_OP_SWAP || op->kind == DQ_OP_RMAT) {
+            if (op->kind == DQ_OP_MAT || op->kind == DQ_OP_SWAP || op->kind == DQ_OP_RMAT || op->kind == DQ_OP_SHEAR) {
                 const uint32_t t = op->tbit, tm = 1u << t;
                 for (uint32_t k2 = tid; k2 < N / 2; k2 += blockDim.x) {
                     const uint32_t i0 = ((k2 >> t) << (t + 1)) | (k2 & (tm - 1));
@@ -594,6 +292,13 @@ small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
                     if ((i0 & c) != v) continue;
                     const V x = s[i0], y = s[i1];
                     if (op->kind == DQ_OP_SWAP) { s[i0] = y; s[i1] = x; continue; }
+                    if (op->kind == DQ_OP_SHEAR) {
+                        V nx, ny;
+                        nx.x = x.x + m[0] * y.x; nx.y = x.y + m[0] * y.y;
+                        ny.x = y.x + m[1] * nx.x; ny.y = y.y + m[1] * nx.y;
+                        s[i0] = nx; s[i1] = ny;
+                        continue;
+                    }
                     V nx, ny;
                     if (op->kind == DQ_OP_RMAT) {
                         nx.x = m[0] * x.x + m[1] * y.x; nx.y = m[0] * x.y + m[1] * y.y;
@@ -744,22 +449,16 @@ __global__ void fill_kernel(typename V2<T>::type *__restrict__ st, uint64_t N, i
 // ----------------------------------------------------------------------------
 // kernel table
 // ----------------------------------------------------------------------------
-typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, const dq_op *);
-
-typedef void (*pipe_fn)(const dq_job *, const dq_block *, const dq_phase *, const dq_op *, uint64_t);
+typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, const unsigned char *, const uint32_t *);
 
 template <typename T>
 struct SweepTable {
     sweep_fn fn[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
-    pipe_fn pipe[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
     SweepTable() {
         memset(fn, 0, sizeof fn);
-        memset(pipe, 0, sizeof pipe);
 #define DQ_INST(R_, W_)                                                                  \
     fn[0][R_][W_] = sweep_kernel<T, R_, W_, false>;                                      \
-    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;                                       \
-    pipe[0][R_][W_] = sweep_pipe_kernel<T, R_, W_, false>;                               \
-    pipe[1][R_][W_] = sweep_pipe_kernel<T, R_, W_, true>;
+    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;
         DQ_INST(1, 0) DQ_INST(2, 0) DQ_INST(3, 0) DQ_INST(4, 0) DQ_INST(5, 0)
         DQ_INST(2, 1) DQ_INST(3, 1) DQ_INST(4, 1) DQ_INST(5, 1)
         DQ_INST(2, 2) DQ_INST(3, 2) DQ_INST(4, 2) DQ_INST(5, 2)
@@ -781,21 +480,6 @@ static bool use_split(int prec, int T) {
     if (g_split_mode == 0) return false;
     if (g_split_mode == 1) return true;
     return prec == DQ_F64 && T >= 12;
-}
-
-// persistent pipelined kernel for single-state launches (DGQ_PIPE=1 enables)
-static int g_pipe_mode = -1;
-static bool use_pipe() {
-    if (g_pipe_mode < 0) {
-        const char *e = getenv("DGQ_PIPE");
-        g_pipe_mode = e ? atoi(e) : 0;
-    }
-    return g_pipe_mode != 0;
-}
-
-static pipe_fn lookup_pipe(int prec, int R, int W, bool split) {
-    if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS) return nullptr;
-    return prec == DQ_F64 ? g_tab64.pipe[split][R][W] : g_tab32.pipe[split][R][W];
 }
 
 static sweep_fn lookup(int prec, int R, int W, bool split = false) {
@@ -853,10 +537,6 @@ extern "C" int dq_device_init(int device) {
                         HIPC(hipFuncSetAttribute((const void *)g_tab64.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
                     if (g_tab32.fn[sp][R][W])
                         HIPC(hipFuncSetAttribute((const void *)g_tab32.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-                    if (g_tab64.pipe[sp][R][W])
-                        HIPC(hipFuncSetAttribute((const void *)g_tab64.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-                    if (g_tab32.pipe[sp][R][W])
-                        HIPC(hipFuncSetAttribute((const void *)g_tab32.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
                 }
         {
             hipDeviceProp_t prop;
@@ -936,9 +616,70 @@ extern "C" int dq_stream_wait_event(void *stream, void *ev) {
 // ----------------------------------------------------------------------------
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Upper bound of the device program built from n_ops descriptors: at most
+// 96 bytes per descriptor (a complex matrix in FP64; a register table of 2^R
+// entries spans 1 + 2^R / 4 descriptors and 32 + 16 * 2^R bytes), plus
+// padding so the interpreter's fixed-size header loads stay in the buffer.
+static size_t prog_bound(uint32_t n_ops) { return (size_t)n_ops * 96 + 256; }
+
 extern "C" size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32_t n_ops, uint32_t n_jobs) {
     return align_up(n_blocks * sizeof(dq_block), 256) + align_up(n_phases * sizeof(dq_phase), 256) +
-           align_up(n_ops * sizeof(dq_op), 256) + align_up(n_jobs * sizeof(dq_job), 256);
+           align_up(n_ops * sizeof(dq_op), 256) + align_up(n_jobs * sizeof(dq_job), 256) +
+           align_up(n_phases * 2 * sizeof(uint32_t), 256) + align_up(prog_bound(n_ops), 256);
+}
+
+// Device program of one phase (format documented in gen_ops_asm.py): one
+// entry per op, a 32-byte header and the coefficients in the state precision,
+// register conditions turned into a mask over register indices.
+static void build_prog(int prec, int R, const dq_op *ops, uint32_t count, std::vector<unsigned char> &out,
+                       uint32_t &n_out) {
+    const size_t ts = prec == DQ_F64 ? 8 : 4;
+    const uint32_t NR = 1u << R;
+    n_out = 0;
+    for (uint32_t k = 0; k < count; k++) {
+        const dq_op &o = ops[k];
+        auto reg_mask = [&](bool pairs) {
+            uint32_t m = 0;
+            for (uint32_t j = 0; j < NR; j++) {
+                if (pairs && ((j >> o.tbit) & 1)) continue;
+                if ((j & o.c_reg) == o.v_reg) m |= 1u << j;
+            }
+            return m;
+        };
+        uint32_t kind = 0, bit = 0, jmask = 0;
+        size_t body = 0;
+        const unsigned char *src = reinterpret_cast<const unsigned char *>(o.m);
+        std::vector<unsigned char> table;
+        switch (o.kind) {
+            case DQ_OP_SHEAR: kind = 1; bit = o.tbit; body = 2 * ts; break;
+            case DQ_OP_DIAGR: {
+                kind = 2;
+                table.resize(NR * 2 * ts);
+                for (uint32_t e = 0; e < NR; e++)
+                    memcpy(table.data() + e * 2 * ts,
+                           reinterpret_cast<const unsigned char *>(ops[k + 1 + e / 4].m) + (e % 4) * 2 * ts, 2 * ts);
+                src = table.data();
+                body = table.size();
+                k += o.tbit;
+                break;
+            }
+            case DQ_OP_NEG: kind = 3; jmask = reg_mask(false); break;
+            case DQ_OP_PHASE: kind = 4; jmask = reg_mask(false); body = 2 * ts; break;
+            case DQ_OP_MAT: kind = 5; bit = o.tbit; jmask = reg_mask(true); body = 8 * ts; break;
+            case DQ_OP_SWAP: kind = 6; bit = o.tbit; jmask = reg_mask(true); break;
+            case DQ_OP_RMAT: kind = 7; bit = o.tbit; body = 4 * ts; break;
+            default: break;
+        }
+        const size_t size = align_up(32 + body, 32);
+        const size_t at = out.size();
+        out.resize(at + size, 0);
+        uint32_t hdr[4] = {kind * 8 + bit, jmask, o.c_thr | (o.v_thr << 16), (uint32_t)size};
+        memcpy(out.data() + at, hdr, 16);
+        memcpy(out.data() + at + 16, &o.c_wg, 8);
+        memcpy(out.data() + at + 24, &o.v_wg, 8);
+        if (body) memcpy(out.data() + at + 32, src, body);
+        n_out++;
+    }
 }
 
 extern "C" int dq_init_state(void *state, int n, int prec, int kind, uint64_t index, double value, void *stream) {
@@ -971,11 +712,16 @@ static int validate_block(const dq_block &b, const dq_phase *phases, uint32_t n_
         if (ph.op_begin > n_ops || ph.op_count > n_ops - ph.op_begin) return fail(DQ_ERR_ARG, "phase: op range out of bounds");
         for (uint32_t k = 0; k < ph.op_count; k++) {
             const dq_op &o = ops[ph.op_begin + k];
+            if (o.c_thr > 0xFFFFu || o.v_thr > 0xFFFFu || (o.v_thr & ~o.c_thr) || (o.v_reg & ~o.c_reg) ||
+                (o.v_wg & ~o.c_wg))
+                return fail(DQ_ERR_ARG, "op: condition values outside their masks");
+            if ((o.kind == DQ_OP_SHEAR || o.kind == DQ_OP_RMAT || o.kind == DQ_OP_DIAGR) && (o.c_reg || o.c_thr))
+                return fail(DQ_ERR_ARG, "op: kind %u takes no register or lane condition", o.kind);
             if (o.kind == DQ_OP_DIAGR) {
                 if (b.reg_bits == 0 || o.tbit != ((1u << b.reg_bits) + 3) / 4 || k + o.tbit >= ph.op_count)
                     return fail(DQ_ERR_ARG, "phase: register-diagonal table out of bounds");
                 k += o.tbit;
-            } else if (o.kind == DQ_OP_MAT || o.kind == DQ_OP_SWAP || o.kind == DQ_OP_RMAT) {
+            } else if (o.kind == DQ_OP_MAT || o.kind == DQ_OP_SWAP || o.kind == DQ_OP_RMAT || o.kind == DQ_OP_SHEAR) {
                 if (b.reg_bits ? o.tbit >= b.reg_bits : o.tbit >= b.n_qubits)
                     return fail(DQ_ERR_ARG, "op: target bit %u out of range", o.tbit);
             } else if (o.kind != DQ_OP_NEG && o.kind != DQ_OP_PHASE) {
@@ -1043,15 +789,37 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             return fail(DQ_ERR_UNSUPPORTED, "no sweep kernel for R=%d W=%d", b0.reg_bits, b0.wave_bits);
     }
 
+    // device programs of the phases of sweep blocks: (byte offset, op count) per phase
+    std::vector<uint32_t> pprog(2 * (size_t)n_phases, 0);
+    std::vector<int> prog_r(n_phases, -1);
+    std::vector<unsigned char> prog;
+    for (uint32_t i = 0; i < n_blocks; i++) {
+        const dq_block &b = blocks[i];
+        if (b.reg_bits == 0) continue;
+        for (uint32_t p = b.phase_begin; p < b.phase_begin + b.phase_count; p++) {
+            if (prog_r[p] == (int)b.reg_bits) continue;
+            if (prog_r[p] >= 0) return fail(DQ_ERR_ARG, "phase %u shared by blocks of different register bits", p);
+            prog_r[p] = (int)b.reg_bits;
+            pprog[2 * p] = (uint32_t)prog.size();
+            build_prog(prec, (int)b.reg_bits, ops + phases[p].op_begin, phases[p].op_count, prog, pprog[2 * p + 1]);
+        }
+    }
+    if (prog.size() + 256 > prog_bound(n_ops)) return fail(DQ_ERR_ARG, "dq_apply_blocks: device program overflow");
+
     // pack and upload
     std::vector<unsigned char> buf(need, 0);
     size_t off_b = 0, off_p = align_up(n_blocks * sizeof(dq_block), 256);
     size_t off_o = off_p + align_up(n_phases * sizeof(dq_phase), 256);
     size_t off_j = off_o + align_up(n_ops * sizeof(dq_op), 256);
+    size_t off_pp = off_j + align_up(n_jobs * sizeof(dq_job), 256);
+    size_t off_pg = off_pp + align_up(n_phases * 2 * sizeof(uint32_t), 256);
     memcpy(buf.data() + off_b, blocks, n_blocks * sizeof(dq_block));
     memcpy(buf.data() + off_p, phases, n_phases * sizeof(dq_phase));
     if (n_ops) memcpy(buf.data() + off_o, ops, n_ops * sizeof(dq_op));
     memcpy(buf.data() + off_j, jobs, n_jobs * sizeof(dq_job));
+    if (n_phases) memcpy(buf.data() + off_pp, pprog.data(), pprog.size() * sizeof(uint32_t));
+    if (!prog.empty()) memcpy(buf.data() + off_pg, prog.data(), prog.size());
+    buf.resize(off_pg + prog.size() + 256, 0);  // upload only what is used (+ padding)
     int rc = stage_upload(d_ws, buf, stream);
     if (rc) return rc;
     unsigned char *d = (unsigned char *)d_ws;
@@ -1059,6 +827,8 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
     const dq_phase *d_phases = (const dq_phase *)(d + off_p);
     const dq_op *d_ops = (const dq_op *)(d + off_o);
     const dq_job *d_jobs = (const dq_job *)(d + off_j);
+    const uint32_t *d_pprog = (const uint32_t *)(d + off_pp);
+    const unsigned char *d_prog = d + off_pg;
 
     for (uint32_t l = 0; l < n_launch; l++) {
         const uint32_t j0 = job_begin[l], nj = job_begin[l + 1] - j0;
@@ -1078,20 +848,9 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             if (lds > 131072) return fail(DQ_ERR_UNSUPPORTED, "tile of %d bits needs %zu bytes of LDS", b0.tile_bits, lds);
             const uint64_t tiles = 1ull << (n - b0.tile_bits);
             if (tiles > 0x7fffffffull || nj > 65535) return fail(DQ_ERR_UNSUPPORTED, "grid too large");
-            if (nj == 1 && use_pipe()) {
-                pipe_fn pf = lookup_pipe(prec, b0.reg_bits, b0.wave_bits, split);
-                int per_cu = 0;
-                HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pf, 64 << b0.wave_bits, lds));
-                if (per_cu < 1) per_cu = 1;
-                uint64_t grid = (uint64_t)per_cu * (uint64_t)g_num_cus;
-                if (grid > tiles) grid = tiles;
-                hipLaunchKernelGGL(pf, dim3((unsigned)grid), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                                   d_jobs + j0, d_blocks, d_phases, d_ops, tiles);
-            } else {
-                sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
-                hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                                   d_jobs + j0, d_blocks, d_phases, d_ops);
-            }
+            sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
+            hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
+                               d_jobs + j0, d_blocks, d_phases, d_prog, d_pprog);
         }
         HIPC(hipGetLastError());
     }

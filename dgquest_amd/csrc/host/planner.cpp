@@ -127,25 +127,50 @@ int reg_of(const TileCtx &c, int q) {
     return c.roles.idx[b];
 }
 
-// Diagonal fusion state of one flush.  The device state equals the logical
-// state times prod_q diag(p0[q], p1[q]) times g: single-qubit diagonals are
-// kept here (they commute with every diagonal and with controls) until the
-// next non-diagonal gate on their qubit absorbs them.
+// Fusion state of one flush.  Invariant: logical state = g * (prod_q P_q) *
+// device state, with P_q = diag(p0[q], p1[q]) * X^x[q] a monomial of unit
+// moduli.  Single-qubit diagonals only update P_q; a diagonal or a control on
+// q only sees x[q] (it flips the condition value); the next non-diagonal gate
+// on q absorbs P_q.  A pending X (x[q] = 1) only lives between two
+// non-diagonal gates on q: the last one of the flush on each qubit leaves a
+// plain diagonal, applied at the end of the flush.
 struct Fuse {
     bool on = false;
     std::vector<cd> p0, p1;
+    std::vector<uint8_t> x;
+    std::vector<int> left;  // non-diagonal gates still to come per target qubit
     cd g = 1.0;
-    explicit Fuse(int n = 0) : p0(n, 1.0), p1(n, 1.0) {}
-    bool trivial(int q) const { return p0[q] == cd(1.0) && p1[q] == cd(1.0); }
+    explicit Fuse(int n = 0) : p0(n, 1.0), p1(n, 1.0), x(n, 0), left(n, 0) {}
+    bool trivial(int q) const { return p0[q] == cd(1.0) && p1[q] == cd(1.0) && !x[q]; }
+    uint64_t flips(uint64_t mask) const {
+        uint64_t f = 0;
+        for (uint64_t m = mask; m; m &= m - 1) {
+            const int q = ctz64(m);
+            if (q < (int)x.size() && x[q]) f |= 1ull << q;
+        }
+        return f;
+    }
 };
 
-// Pending real rotations of one phase: their right diagonals diag(1, r1) are
-// applied together as one register-indexed diagonal, then the rotations.
+// Pending factored gates of one phase (at most one per qubit): their right
+// diagonals diag(d0, d1) are applied together as one register-indexed table,
+// then the real parts (SHEAR, or RMAT for the last gate on a qubit).
 struct Batch {
     std::vector<int> reg;
-    std::vector<std::array<double, 4>> M;
-    std::vector<cd> r1;
+    std::vector<uint32_t> kind;
+    std::vector<std::array<double, 4>> m;
+    std::vector<cd> d0, d1;
     uint64_t qmask = 0;
+    void add(int r, uint32_t k, const double *c, int nc, cd a0, cd a1, int q) {
+        std::array<double, 4> v{};
+        for (int i = 0; i < nc; i++) v[i] = c[i];
+        reg.push_back(r);
+        kind.push_back(k);
+        m.push_back(v);
+        d0.push_back(a0);
+        d1.push_back(a1);
+        qmask |= 1ull << q;
+    }
 };
 
 void emit_table(OpList &L, const TileCtx &c, const std::vector<cd> &table) {
@@ -164,34 +189,31 @@ void emit_table(OpList &L, const TileCtx &c, const std::vector<cd> &table) {
     }
 }
 
-void flush_batch(Batch &b, OpList &L, const TileCtx &c) {
+void flush_batch(Batch &b, OpList &L, const TileCtx &c, Fuse &F) {
     if (b.reg.empty()) return;
     const int n_ent = 1 << c.R;
-    std::vector<cd> table(n_ent, 1.0);
+    std::vector<cd> table(n_ent, F.g);
+    F.g = 1.0;
+    for (size_t i = 0; i < b.reg.size(); i++)
+        for (int j = 0; j < n_ent; j++) table[j] *= ((j >> b.reg[i]) & 1) ? b.d1[i] : b.d0[i];
     bool any = false;
-    for (size_t i = 0; i < b.reg.size(); i++) {
-        if (b.r1[i] == cd(1.0)) continue;
-        any = true;
-        for (int j = 0; j < n_ent; j++)
-            if ((j >> b.reg[i]) & 1) table[j] *= b.r1[i];
-    }
+    for (const cd &t : table) any |= t != cd(1.0);
     if (any) emit_table(L, c, table);
     dq_op z;
     memset(&z, 0, sizeof z);
-    for (size_t i = 0; i < b.reg.size(); i++) {
-        const double m[4] = {b.M[i][0], b.M[i][1], b.M[i][2], b.M[i][3]};
-        L.add(DQ_OP_RMAT, (uint32_t)b.reg[i], z, m, 4);
-    }
+    for (size_t i = 0; i < b.reg.size(); i++)
+        L.add(b.kind[i], (uint32_t)b.reg[i], z, b.m[i].data(), b.kind[i] == DQ_OP_SHEAR ? 2 : 4);
     b = Batch();
 }
 
-// Apply the pending diagonal of qubit q now.
+// Apply the diagonal part of P_q now (x[q] must be 0).
 void apply_pending(Fuse &F, int q, OpList &L, const TileCtx &c) {
-    if (F.trivial(q)) return;
+    if (F.p0[q] == cd(1.0) && F.p1[q] == cd(1.0)) return;
     const int r = reg_of(c, q);
     if (r >= 0) {
         std::vector<cd> table(1 << c.R);
-        for (int j = 0; j < (1 << c.R); j++) table[j] = ((j >> r) & 1) ? F.p1[q] : F.p0[q];
+        for (int j = 0; j < (1 << c.R); j++) table[j] = F.g * (((j >> r) & 1) ? F.p1[q] : F.p0[q]);
+        F.g = 1.0;
         emit_table(L, c, table);
     } else {
         F.g *= F.p0[q];
@@ -250,39 +272,94 @@ bool real_factor(const cd U[4], cd &l0, cd &l1, cd &r1, double M[4]) {
     return true;
 }
 
-void emit_general(OpList &L, const TileCtx &c, bool small, uint32_t kind, uint32_t tbit, uint64_t cm, const double *m) {
-    L.add(kind, tbit, cond_of(c, small, cm, cm), m, 8);
+// Sheared form of a 2x2 unitary M (times unit-modulus monomials):
+//   M = diag(l0, l1) X^x * L(t) * U(u) * diag(d0, d1),
+//   U(u) = [[1, u], [0, 1]], L(t) = [[1, 0], [t, 1]], |t| <= 1, |u| <= 1/2.
+// From M = diag(l0, l1) R(th) diag(1, r) with R(th) = [[c, -s], [s, c]]:
+//   c >= s:  R = L(s/c) U(-s c) diag(c, 1/c)
+//   s >  c:  R = diag(-1, 1) X R(th - pi/2), the same form with (c, s) -> (s, -c).
+// The device applies diag(d0, d1) (in a register table), then the two shears
+// (4 FMAs per amplitude pair, in place); diag(l0, l1) X^x stays pending.
+// kind: 0 general, 1 M is diagonal, 2 M is anti-diagonal (no device op then).
+// Returns false if rebuilding M from the factors misses by more than rounding.
+bool shear_factor(const cd M[4], int &kind, cd &l0, cd &l1, int &x, double &u, double &t, cd &d0, cd &d1) {
+    const double c = std::abs(M[0]), s = std::abs(M[2]);
+    x = 0; u = t = 0; d0 = d1 = 1.0;
+    if (s == 0.0 && std::abs(M[1]) == 0.0) {
+        kind = 1; l0 = M[0]; l1 = M[3];
+        return std::abs(std::abs(l0) - 1.0) < 1e-14 && std::abs(std::abs(l1) - 1.0) < 1e-14;
+    }
+    if (c == 0.0 && std::abs(M[3]) == 0.0) {
+        kind = 2; l0 = M[1]; l1 = M[2]; x = 1;
+        return std::abs(std::abs(l0) - 1.0) < 1e-14 && std::abs(std::abs(l1) - 1.0) < 1e-14;
+    }
+    kind = 0;
+    if (c == 0.0 || s == 0.0) return false;  // not unitary
+    l0 = M[0] / c;
+    l1 = M[2] / s;
+    const cd r = (c >= s) ? M[3] / (l1 * c) : -M[1] / (l0 * s);
+    double cc = c, ss = s;
+    if (c >= s) {
+        x = 0;
+    } else {  // diag(l0, l1) diag(-1, 1) X: pending -l0 on the |0> row
+        x = 1;
+        l0 = -l0;
+        cc = s;
+        ss = -c;
+    }
+    t = ss / cc;
+    u = -ss * cc;
+    d0 = cc;
+    d1 = r / cc;
+    // rebuild: diag(l0, l1) X^x L(t) U(u) diag(d0, d1)
+    const cd S[4] = {d0, u * d1, t * d0, (1.0 + t * u) * d1};
+    cd R4[4];
+    if (!x) {
+        R4[0] = l0 * S[0]; R4[1] = l0 * S[1]; R4[2] = l1 * S[2]; R4[3] = l1 * S[3];
+    } else {
+        R4[0] = l0 * S[2]; R4[1] = l0 * S[3]; R4[2] = l1 * S[0]; R4[3] = l1 * S[1];
+    }
+    for (int i = 0; i < 4; i++)
+        if (std::abs(R4[i] - M[i]) > 8e-16 * (1.0 + std::abs(M[i]))) return false;
+    return true;
 }
 
-// Diagonal gates with two or more qubits (or a control): emitted as they come.
+void emit_general(OpList &L, const TileCtx &c, bool small, uint32_t kind, uint32_t tbit, uint64_t cm, uint64_t cv,
+                  const double *m) {
+    L.add(kind, tbit, cond_of(c, small, cm, cv), m, 8);
+}
+
+// Diagonal gates with two or more qubits (or a control): emitted as they come,
+// condition values flipped on qubits with a pending X.
 void emit_diagonal(const Gate &g, OpList &L, Fuse &F, const TileCtx &c, bool small) {
     const uint64_t cm = g.cmask;
+    const uint64_t fl = F.flips(g.cmask | g.tmask);
     switch (g.type) {
         case OP_ONE: return;
-        case OP_PAULI_Z: L.add(DQ_OP_NEG, 0, cond_of(c, small, cm, cm), nullptr, 0); return;
+        case OP_PAULI_Z: L.add(DQ_OP_NEG, 0, cond_of(c, small, cm, cm ^ fl), nullptr, 0); return;
         case OP_PHASE_SHIFT: {
             const double p[2] = {std::cos(g.op[0]), std::sin(g.op[0])};
-            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm, cm), p, 2);
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm, cm ^ fl), p, 2);
             return;
         }
         case OP_ROTATE_Z: {  // controlled: diag(e^{-ia}, e^{ia}) on the target where the controls are 1
             const double a = g.op[0];
             const uint64_t t = g.tmask;
             const double p0[2] = {std::cos(a), -std::sin(a)}, p1[2] = {std::cos(a), std::sin(a)};
-            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm), p0, 2);
-            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm | t), p1, 2);
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm ^ fl), p0, 2);
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, (cm | t) ^ fl), p1, 2);
             return;
         }
         case OP_COLLAPSE: {  // keep the outcome branch scaled by 1/sqrt(P), zero the other
             const int outcome = (int)g.op[0];
             const uint64_t t = g.tmask;
             const double zero[2] = {0, 0};
-            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm | (outcome ? 0 : t)), zero, 2);
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, (cm | (outcome ? 0 : t)) ^ fl), zero, 2);
             if (cm == 0 && F.on) {
                 F.g *= g.op[1];
             } else {
                 const double sc[2] = {g.op[1], 0};
-                L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, cm | (outcome ? t : 0)), sc, 2);
+                L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm | t, (cm | (outcome ? t : 0)) ^ fl), sc, 2);
             }
             return;
         }
@@ -290,7 +367,7 @@ void emit_diagonal(const Gate &g, OpList &L, Fuse &F, const TileCtx &c, bool sma
             static const double pr[8] = {1, kRsqrt2, 0, -kRsqrt2, -1, -kRsqrt2, 0, kRsqrt2};
             static const double pi[8] = {0, kRsqrt2, 1, kRsqrt2, 0, -kRsqrt2, -1, -kRsqrt2};
             const double p[2] = {pr[g.type], pi[g.type]};
-            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm, cm), p, 2);
+            L.add(DQ_OP_PHASE, 0, cond_of(c, small, cm, cm ^ fl), p, 2);
             return;
         }
     }
@@ -316,10 +393,26 @@ bool single_diag(const Gate &g, int &q, cd &d0, cd &d1) {
     return true;
 }
 
+// U * P_q as a complex 2x2 (P_q = diag(p0, p1) X^x)
+void absorb(const double m[8], const Fuse &F, int t, cd M[4]) {
+    const cd U[4] = {cd(m[0], m[1]), cd(m[2], m[3]), cd(m[4], m[5]), cd(m[6], m[7])};
+    M[0] = U[0] * F.p0[t]; M[1] = U[1] * F.p1[t];
+    M[2] = U[2] * F.p0[t]; M[3] = U[3] * F.p1[t];
+    if (F.x[t]) { std::swap(M[0], M[1]); std::swap(M[2], M[3]); }
+}
+
+void flat(const cd M[4], double m[8]) {
+    for (int i = 0; i < 4; i++) { m[2 * i] = M[i].real(); m[2 * i + 1] = M[i].imag(); }
+}
+
+bool is_x(const cd M[4]) { return M[0] == cd(0.0) && M[3] == cd(0.0) && M[1] == cd(1.0) && M[2] == cd(1.0); }
+
 // Ops of one phase.  With fusion on, uncontrolled single-qubit gates become
-// real rotations plus pending diagonals.
+// sheared rotations plus pending monomials (see Fuse and shear_factor).
 void pack_phase(const std::vector<Gate> &gates, Fuse &F, OpList &L, const TileCtx &c, bool small) {
     Batch b;
+    dq_op z;
+    memset(&z, 0, sizeof z);
     for (const Gate &g : gates) {
         const uint64_t touched = g.tmask | g.cmask;
         if (gate_is_diagonal(g.type)) {
@@ -330,54 +423,80 @@ void pack_phase(const std::vector<Gate> &gates, Fuse &F, OpList &L, const TileCt
                 F.p1[q] *= d1;
                 continue;
             }
-            if (b.qmask & touched) flush_batch(b, L, c);
+            if (b.qmask & touched) flush_batch(b, L, c, F);
             emit_diagonal(g, L, F, c, small);
             continue;
         }
         const int t = ctz64(g.tmask);
         const uint32_t tb = small ? (uint32_t)t : (uint32_t)reg_of(c, t);
-        if (!F.on) {
-            double m[8];
-            gate_matrix(g, m);
-            emit_general(L, c, small, g.type == OP_PAULI_X ? DQ_OP_SWAP : DQ_OP_MAT, tb, g.cmask, m);
-            continue;
-        }
-        if (b.qmask & touched) flush_batch(b, L, c);
-        if (g.cmask) {
-            apply_pending(F, t, L, c);
-            double m[8];
-            gate_matrix(g, m);
-            emit_general(L, c, small, g.type == OP_PAULI_X ? DQ_OP_SWAP : DQ_OP_MAT, tb, g.cmask, m);
-            continue;
-        }
-        if (g.type == OP_PAULI_X) {  // X diag(p0, p1) = diag(p1, p0) X
-            std::swap(F.p0[t], F.p1[t]);
-            dq_op z;
-            memset(&z, 0, sizeof z);
-            L.add(DQ_OP_SWAP, tb, z, nullptr, 0);
-            continue;
-        }
         double m[8];
         gate_matrix(g, m);
-        const cd U[4] = {cd(m[0], m[1]) * F.p0[t], cd(m[2], m[3]) * F.p1[t], cd(m[4], m[5]) * F.p0[t],
-                         cd(m[6], m[7]) * F.p1[t]};
-        cd l0, l1, r1;
-        double M[4];
-        if (real_factor(U, l0, l1, r1, M)) {
-            b.reg.push_back((int)tb);
-            b.M.push_back({M[0], M[1], M[2], M[3]});
-            b.r1.push_back(r1);
-            b.qmask |= 1ull << t;
-            F.p0[t] = l0;
-            F.p1[t] = l1;
-        } else {
-            const double mu[8] = {U[0].real(), U[0].imag(), U[1].real(), U[1].imag(),
-                                  U[2].real(), U[2].imag(), U[3].real(), U[3].imag()};
-            F.p0[t] = F.p1[t] = 1.0;
-            emit_general(L, c, small, DQ_OP_MAT, tb, 0, mu);
+        if (!F.on) {
+            emit_general(L, c, small, g.type == OP_PAULI_X ? DQ_OP_SWAP : DQ_OP_MAT, tb, g.cmask, g.cmask, m);
+            continue;
         }
+        if (b.qmask & touched) flush_batch(b, L, c, F);
+        const bool last = --F.left[t] <= 0;
+        if (g.cmask) {
+            // C-U * P_t = P_t * C-(P_t^-1 U P_t); controls with a pending X flip
+            const uint64_t cv = g.cmask ^ F.flips(g.cmask);
+            if (last && F.x[t]) {  // the last gate on t leaves no pending X
+                L.add(DQ_OP_SWAP, tb, z, nullptr, 0);
+                F.x[t] = 0;
+            }
+            cd U[4] = {cd(m[0], m[1]), cd(m[2], m[3]), cd(m[4], m[5]), cd(m[6], m[7])};
+            const cd a0 = F.p0[t], a1 = F.p1[t];
+            cd V[4] = {U[0], U[1] * a1 / a0, U[2] * a0 / a1, U[3]};  // D^-1 U D
+            if (F.x[t]) { std::swap(V[0], V[3]); std::swap(V[1], V[2]); }  // X V X
+            double mv[8];
+            flat(V, mv);
+            emit_general(L, c, small, is_x(V) ? DQ_OP_SWAP : DQ_OP_MAT, tb, g.cmask, cv, mv);
+            continue;
+        }
+        if (g.type == OP_PAULI_X) {  // X diag(p0, p1) X^x = diag(p1, p0) X^(x+1)
+            std::swap(F.p0[t], F.p1[t]);
+            F.x[t] ^= 1;
+            if (last && F.x[t]) {
+                L.add(DQ_OP_SWAP, tb, z, nullptr, 0);
+                F.x[t] = 0;
+            }
+            continue;
+        }
+        cd M[4];
+        absorb(m, F, t, M);
+        if (!last) {
+            int kind, x;
+            cd l0, l1, d0, d1;
+            double u, tt;
+            if (shear_factor(M, kind, l0, l1, x, u, tt, d0, d1)) {
+                F.p0[t] = l0;
+                F.p1[t] = l1;
+                F.x[t] = (uint8_t)x;
+                if (kind == 0) {
+                    const double sh[2] = {u, tt};
+                    b.add((int)tb, DQ_OP_SHEAR, sh, 2, d0, d1, t);
+                }
+                continue;
+            }
+        } else {
+            cd l0, l1, r1;
+            double R4[4];
+            if (real_factor(M, l0, l1, r1, R4)) {
+                F.p0[t] = l0;
+                F.p1[t] = l1;
+                F.x[t] = 0;
+                b.add((int)tb, DQ_OP_RMAT, R4, 4, 1.0, r1, t);
+                continue;
+            }
+        }
+        // fall back to the complex matrix (absorbs the pending monomial)
+        double mu[8];
+        flat(M, mu);
+        F.p0[t] = F.p1[t] = 1.0;
+        F.x[t] = 0;
+        emit_general(L, c, small, DQ_OP_MAT, tb, 0, 0, mu);
     }
-    flush_batch(b, L, c);
+    flush_batch(b, L, c, F);
 }
 
 Layout standard_layout(int T, int R, int W) {
@@ -577,6 +696,9 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         bgs.emplace_back();
         tms.push_back(0x3Full);
     }
+    for (const auto &bg : bgs)
+        for (const Gate &g : bg)
+            if (!gate_is_diagonal(g.type)) F.left[ctz64(g.tmask)]++;
     for (size_t i = 0; i < bgs.size(); i++)
         emit_block(bgs[i], tms[i], n, t, prec, i == 0 ? init_kind : DQ_INIT_LOAD, init_index, init_value, F,
                    i + 1 == bgs.size(), P);

@@ -243,6 +243,7 @@ static void run_plan(QuregState *s, const Plan &P) {
 }
 
 void flush(QuregState *s) {
+    if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
     if (s->pending.empty() && s->init_kind == DQ_INIT_LOAD) return;
     const auto t0 = std::chrono::steady_clock::now();
     const unsigned long long state_bytes = (unsigned long long)(s->prec == DQ_F64 ? 16 : 8) << s->n;
@@ -432,6 +433,10 @@ void destroyQureg(Qureg qureg, QuESTEnv env) {
     (void)env;
     QuregState *s = st(qureg);
     if (!s) return;
+    if (s->recorder) {
+        delete s;
+        return;
+    }
     dq_stream_sync(s->stream);
     dq_free(s->d_state);
     if (s->d_ws) dq_free(s->d_ws);
@@ -480,6 +485,7 @@ void initClassicalState(Qureg qureg, long long int stateInd) {
 void initStateFromAmps(Qureg qureg, qreal *reals, qreal *imags) {
     v_statevec(qureg, __func__);
     QuregState *s = st(qureg);
+    if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
     set_init(qureg, DQ_INIT_LOAD, 0, 0.0);
     const uint64_t N = qureg.numAmpsTotal;
     const uint64_t chunk = 1ull << 22;
@@ -884,6 +890,52 @@ int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k,
     stats[5] = t.k;
     stats[6] = t.R;
     stats[7] = t.W;
+    return 0;
+}
+
+Qureg dgq_create_recorder(int numQubits) {
+    quest_assert(numQubits > 0 && numQubits <= 40, E_INVALID_NUM_QUBITS, __func__);
+    Qureg q;
+    memset(&q, 0, sizeof q);
+    q.numQubitsRepresented = (u64)numQubits;
+    q.numQubitsInStateVec = (u64)numQubits;
+    q.numAmpsTotal = 1ull << numQubits;
+    QuregState *s = new QuregState();
+    s->n = numQubits;
+    s->prec = sizeof(qreal) == 8 ? DQ_F64 : DQ_F32;
+    s->tune = env_tune(s->prec);
+    s->recorder = true;
+    s->l2p.resize(numQubits);
+    s->p2l.resize(numQubits);
+    for (int i = 0; i < numQubits; i++) s->l2p[i] = s->p2l[i] = i;
+    s->init_kind = DQ_INIT_CLASSICAL;  // a new register is |0...0>
+    q.state = s;
+    return q;
+}
+
+void dgq_destroy_recorder(Qureg q) { destroyQureg(q, q.env); }
+
+int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_block *blocks, int cap_blocks,
+                      dq_phase *phases, int cap_phases, dq_op *ops, int cap_ops, int counts[3], int *l2p) {
+    QuregState *s = q.state;
+    if (!s || !s->recorder || !counts) return -1;
+    TuneParams tp;
+    tp.k = k;
+    tp.R = R;
+    tp.W = W;
+    tp.max_ops = max_ops;
+    tp.fuse_diagonals = fuse != 0;
+    Plan P;
+    plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, tp, P);
+    counts[0] = (int)P.blocks.size();
+    counts[1] = (int)P.phases.size();
+    counts[2] = (int)P.ops.size();
+    if (l2p)
+        for (int i = 0; i < s->n; i++) l2p[i] = s->l2p[i];
+    if (counts[0] > cap_blocks || counts[1] > cap_phases || counts[2] > cap_ops) return 1;
+    if (counts[0]) memcpy(blocks, P.blocks.data(), P.blocks.size() * sizeof(dq_block));
+    if (counts[1]) memcpy(phases, P.phases.data(), P.phases.size() * sizeof(dq_phase));
+    if (counts[2]) memcpy(ops, P.ops.data(), P.ops.size() * sizeof(dq_op));
     return 0;
 }
 

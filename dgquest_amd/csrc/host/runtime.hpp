@@ -39,6 +39,7 @@ struct QuregState {
     unsigned long long last_bytes = 0;  // algorithmic HBM bytes of the last flush
     int last_launches = 0;
     double last_flush_ms = 0.0;
+    bool recorder = false;            // records and plans only, no device state
 };
 
 namespace dgq {

@@ -71,6 +71,28 @@ GATE_SIGNATURES = {
 _NREALS = {"r": 1, "c": 2, "m": 8, "v": 3}
 
 
+def _descriptor_dtypes():
+    """numpy layouts of dq_block, dq_phase and dq_op (include/dgq_engine.h)."""
+    import numpy as np
+
+    block = np.dtype([("n_qubits", "<u4"), ("tile_bits", "<u4"), ("reg_bits", "<u4"), ("wave_bits", "<u4"),
+                      ("init_kind", "<u4"), ("phase_begin", "<u4"), ("phase_count", "<u4"), ("pad0", "<u4"),
+                      ("nontile_mask", "<u8"), ("init_index", "<u8"), ("init_value", "<f8"), ("pad1", "<u8"),
+                      ("ld_reg", "<u8", 5), ("ld_wave", "<u8", 3), ("st_reg", "<u8", 5), ("st_wave", "<u8", 3)])
+    phase = np.dtype([("reg_t", "<u4", 5), ("wave_t", "<u4", 3), ("lane_t", "<u4", 6), ("op_begin", "<u4"),
+                      ("op_count", "<u4")])
+    op = np.dtype([("kind", "<u4"), ("tbit", "<u4"), ("c_reg", "<u4"), ("v_reg", "<u4"), ("c_thr", "<u4"),
+                   ("v_thr", "<u4"), ("c_wg", "<u8"), ("v_wg", "<u8"), ("m", "<f8", 8)])
+    assert block.itemsize == 192 and phase.itemsize == 64 and op.itemsize == 104
+    return block, phase, op
+
+
+try:
+    BLOCK_DT, PHASE_DT, OP_DT = _descriptor_dtypes()
+except ImportError:  # numpy is only needed for the plan export
+    BLOCK_DT = PHASE_DT = OP_DT = None
+
+
 class QuEST:
     """One loaded precision of the QuEST API (``prec`` 2 = double, 1 = float)."""
 
@@ -141,6 +163,14 @@ class QuEST:
         L.dgq_flush.argtypes = [Qureg]
         L.dgq_read_state.restype = C.c_int
         L.dgq_read_state.argtypes = [Qureg, C.c_ulonglong, C.c_ulonglong, C.POINTER(C.c_double)]
+        L.dgq_create_recorder.restype = Qureg
+        L.dgq_create_recorder.argtypes = [C.c_int]
+        L.dgq_destroy_recorder.restype = None
+        L.dgq_destroy_recorder.argtypes = [Qureg]
+        L.dgq_recorder_plan.restype = C.c_int
+        L.dgq_recorder_plan.argtypes = [Qureg, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
 
     # --- thin pass-throughs -------------------------------------------------
     def __getattr__(self, name):
@@ -210,6 +240,27 @@ class QuEST:
         l = C.c_int()
         self.lib.dgq_last_flush_bytes(q, C.byref(b), C.byref(l))
         return b.value, l.value
+
+    def recorder_plan(self, q: Qureg, k=6, R=4, W=2, max_ops=4096, fuse=True):
+        """The plan of what the recording register q holds, as numpy structured
+        arrays (blocks, phases, ops) in the layout of include/dgq_engine.h,
+        plus the logical -> physical qubit map."""
+        import numpy as np
+
+        n = int(q.numQubitsRepresented)
+        counts = (C.c_int * 3)()
+        l2p = (C.c_int * n)()
+        rc = self.lib.dgq_recorder_plan(q, k, R, W, max_ops, int(fuse), None, 0, None, 0, None, 0, counts, l2p)
+        if rc < 0:
+            raise RuntimeError("dgq_recorder_plan failed")
+        b = np.zeros(counts[0], dtype=BLOCK_DT)
+        p = np.zeros(counts[1], dtype=PHASE_DT)
+        o = np.zeros(counts[2], dtype=OP_DT)
+        rc = self.lib.dgq_recorder_plan(q, k, R, W, max_ops, int(fuse), b.ctypes.data, len(b), p.ctypes.data, len(p),
+                                        o.ctypes.data, len(o), counts, l2p)
+        if rc != 0:
+            raise RuntimeError("dgq_recorder_plan failed")
+        return b, p, o, list(l2p)
 
     def plan_stats(self, q: Qureg):
         """Statistics of the last flush: sweeps, phases, exchanges, ops, gates_in, k, R, W + flush ms."""

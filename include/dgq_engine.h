@@ -76,6 +76,9 @@ typedef struct dq_phase {
 #define DQ_OP_RMAT 6  /* real 2x2 matrix [[m0, m1], [m2, m3]] on register bit tbit */
 #define DQ_OP_DIAGR 7 /* amplitude of register index j *= table[j]; the table (2^R complex)
                        * follows in the next tbit ops, 4 complex (re, im) per op in m[] */
+#define DQ_OP_SHEAR 8 /* two real shears on register bit tbit, in place: x += m0 * y, then
+                       * y += m1 * x (x = amplitude with the bit 0, y = with the bit 1);
+                       * 4 FMAs per pair, the real part of a factored 1-qubit gate */
 
 /* Conditions: an op acts on an amplitude only if
  *   (reg_index & c_reg) == v_reg, (threadIdx.x & c_thr) == v_thr and

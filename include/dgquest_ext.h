@@ -11,6 +11,7 @@
 #define DGQUEST_EXT_H
 
 #include "QuEST.h"
+#include "dgq_engine.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -62,6 +63,25 @@ typedef struct dgq_gate_rec {
 } dgq_gate_rec;
 int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k, int R, int W, int max_ops,
                   int stats[8]);
+
+/* A register that only records and plans, with no device state (no GPU
+ * needed): the gate calls of include/QuEST.h validate and record on it as on
+ * any register, so the front end and the planner can be checked on any host.
+ * Anything that would execute or read the state (flush, probabilities,
+ * amplitudes, initStateFromAmps, measurement) ends the program with an
+ * error on such a register. */
+Qureg dgq_create_recorder(int numQubits);
+void dgq_destroy_recorder(Qureg q);
+
+/* Plan what q has recorded (pending initialisation included) with the given
+ * shape and diagonal fusion on/off, and copy the plan into the caller's
+ * arrays (C ABI of include/dgq_engine.h).  counts[0..2] receive the numbers of
+ * blocks, phases and ops; l2p (n ints, may be NULL) the logical -> physical
+ * qubit map the plan leaves.  Returns 0, 1 when a capacity is too small
+ * (counts then give the sizes; nothing else is written), -1 on bad
+ * arguments.  The recording is left as it is. */
+int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_block *blocks, int cap_blocks,
+                      dq_phase *phases, int cap_phases, dq_op *ops, int cap_ops, int counts[3], int *l2p);
 
 #ifdef __cplusplus
 }
