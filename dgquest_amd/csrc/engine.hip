@@ -652,12 +652,24 @@ static void build_prog(int prec, int R, const dq_op *ops, uint32_t count, std::v
         std::vector<unsigned char> table;
         switch (o.kind) {
             case DQ_OP_SHEAR: kind = 1; bit = o.tbit; body = 2 * ts; break;
-            case DQ_OP_DIAGR: {
+            case DQ_OP_DIAGR: {  // jmask: the entries that are not exactly 1
                 kind = 2;
                 table.resize(NR * 2 * ts);
-                for (uint32_t e = 0; e < NR; e++)
-                    memcpy(table.data() + e * 2 * ts,
-                           reinterpret_cast<const unsigned char *>(ops[k + 1 + e / 4].m) + (e % 4) * 2 * ts, 2 * ts);
+                for (uint32_t e = 0; e < NR; e++) {
+                    const unsigned char *ent = reinterpret_cast<const unsigned char *>(ops[k + 1 + e / 4].m) + (e % 4) * 2 * ts;
+                    memcpy(table.data() + e * 2 * ts, ent, 2 * ts);
+                    bool one;
+                    if (prec == DQ_F64) {
+                        double v[2];
+                        memcpy(v, ent, sizeof v);
+                        one = v[0] == 1.0 && v[1] == 0.0;
+                    } else {
+                        float v[2];
+                        memcpy(v, ent, sizeof v);
+                        one = v[0] == 1.0f && v[1] == 0.0f;
+                    }
+                    if (!one) jmask |= 1u << e;
+                }
                 src = table.data();
                 body = table.size();
                 k += o.tbit;

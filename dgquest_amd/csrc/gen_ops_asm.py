@@ -15,7 +15,7 @@ each op is a 32-byte header followed by coefficients of the amplitude type,
 padded to a multiple of 32 bytes:
     u32 disp    kind * 8 + register bit
     u32 jmask   register indices j that satisfy the op's register condition
-                (pair ops: the lower index of each pair)
+                (pair ops: the lower index of each pair; DIAGR: entries != 1)
     u32 thr     lane condition: c_thr | v_thr << 16 (c_thr = 0: every lane)
     u32 size    bytes of this op, header included
     u64 c_wg    workgroup condition: (tile base & c_wg) == v_wg
@@ -114,13 +114,25 @@ class Gen:
     def diagr(self):
         # table at current op + 32, read in 64-byte chunks into the header
         # buffer (the header is no longer needed once the op is dispatched)
+        # (jmask, moved to s[SCR] before the header buffer is reused, skips the
+        # registers whose factor is exactly 1)
         per = 4 if self.d else 8          # complex entries per 64-byte chunk
         nch = (self.NR + per - 1) // per
+        f = self.sfx
+        self.e(f"s_mov_b32 s{SCR}, s{HDR + 1}")
         for c in range(nch):
             self.e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], {sp(CUR)}, {hex(0x20 + 64 * c)}")
             self.e("s_waitcnt lgkmcnt(0)")
-            js = list(range(c * per, min(self.NR, (c + 1) * per)))
-            self.cmul(js, lambda j: self.sbuf(HDR, 2 * (j % per)), lambda j: self.sbuf(HDR, 2 * (j % per) + 1))
+            for j in range(c * per, min(self.NR, (c + 1) * per)):
+                pr, pi = self.sbuf(HDR, 2 * (j % per)), self.sbuf(HDR, 2 * (j % per) + 1)
+                tag = f"dg{j}"
+                self.e(f"s_bitcmp1_b32 s{SCR}, {j}")
+                self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
+                self.e(f"v_mul_{f} %[t0], {pi}, {self.im(j)}")
+                self.e(f"v_mul_{f} %[t1], {pi}, {self.re(j)}")
+                self.e(f"v_fma_{f} {self.re(j)}, {pr}, {self.re(j)}, -%[t0]")
+                self.e(f"v_fma_{f} {self.im(j)}, {pr}, {self.im(j)}, %[t1]")
+                self.e(f"{self.lbl(tag)}:")
 
     def jskip(self, j, tag):
         self.e(f"s_bitcmp1_b32 s{HDR + 1}, {j}")

@@ -189,13 +189,26 @@ void emit_table(OpList &L, const TileCtx &c, const std::vector<cd> &table) {
     }
 }
 
+// The table of a batch holds diag(1, d1 / d0) per gate (the device skips
+// entries that are exactly 1, half the registers for one gate); the d0 go to
+// the global factor, folded into a table once it drifts far from 1 (the
+// device state is the logical one over the global factor).
 void flush_batch(Batch &b, OpList &L, const TileCtx &c, Fuse &F) {
     if (b.reg.empty()) return;
     const int n_ent = 1 << c.R;
-    std::vector<cd> table(n_ent, F.g);
-    F.g = 1.0;
+    for (size_t i = 0; i < b.reg.size(); i++) {
+        F.g *= b.d0[i];
+        b.d1[i] /= b.d0[i];
+    }
+    const double ag = std::abs(F.g);
+    std::vector<cd> table(n_ent, 1.0);
+    if (ag < 0x1p-20 || ag > 0x1p20) {
+        for (auto &t : table) t = F.g;
+        F.g = 1.0;
+    }
     for (size_t i = 0; i < b.reg.size(); i++)
-        for (int j = 0; j < n_ent; j++) table[j] *= ((j >> b.reg[i]) & 1) ? b.d1[i] : b.d0[i];
+        for (int j = 0; j < n_ent; j++)
+            if ((j >> b.reg[i]) & 1) table[j] *= b.d1[i];
     bool any = false;
     for (const cd &t : table) any |= t != cd(1.0);
     if (any) emit_table(L, c, table);
@@ -212,8 +225,8 @@ void apply_pending(Fuse &F, int q, OpList &L, const TileCtx &c) {
     const int r = reg_of(c, q);
     if (r >= 0) {
         std::vector<cd> table(1 << c.R);
-        for (int j = 0; j < (1 << c.R); j++) table[j] = F.g * (((j >> r) & 1) ? F.p1[q] : F.p0[q]);
-        F.g = 1.0;
+        F.g *= F.p0[q];
+        for (int j = 0; j < (1 << c.R); j++) table[j] = ((j >> r) & 1) ? F.p1[q] / F.p0[q] : cd(1.0);
         emit_table(L, c, table);
     } else {
         F.g *= F.p0[q];
