@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--shape", type=str, default="", help="k,R,W to skip the autotuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--paged-bv", type=int, default=28,
+                    help="qubits of the host-paged Bernstein-Vazirani run (0 = skip)")
+    ap.add_argument("--paged-budget-gib", type=float, default=1.0)
     return ap.parse_args()
 
 
@@ -92,6 +95,60 @@ def cpu_baseline(circ, n, seconds):
         "sample": f"first {done} of {len(circ)} gates of the same {n}-qubit circuit, FP64, gate by gate "
                   f"(one state read+write per gate) in {dt:.1f} s; whole circuit extrapolates to "
                   f"{dt / done * len(circ):.0f} s",
+    }
+
+
+def paged_bv(q64, env, n_data, budget_bytes, seed=3):
+    """Config-3 shape: Bernstein-Vazirani on n_data + 1 qubits held in pinned
+    host pieces and paged through a device budget of `budget_bytes`.  Returns
+    the recovered-string check and the copy/compute overlap of the flush."""
+    import ctypes as C
+
+    from dgquest_amd import circuits
+
+    circ, hidden = circuits.bernstein_vazirani(n_data, seed)
+    old = os.environ.get("DGQ_HBM_BUDGET")
+    os.environ["DGQ_HBM_BUDGET"] = str(int(budget_bytes))
+    try:
+        q = q64.createQureg(n_data + 1, env)
+    finally:
+        if old is None:
+            del os.environ["DGQ_HBM_BUDGET"]
+        else:
+            os.environ["DGQ_HBM_BUDGET"] = old
+    L = q64.lib
+    L.dgq_paged_info.restype = C.c_int
+    L.dgq_paged_info.argtypes = [type(q), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    nl, K = C.c_int(), C.c_int()
+    st = (C.c_double * 10)()
+    if L.dgq_paged_info(q, C.byref(nl), C.byref(K), st) != 0:
+        raise RuntimeError("register is not paged")
+    t0 = time.perf_counter()
+    q64.apply_circuit(q, circ)
+    q64.dgq_flush(q)
+    flush_s = time.perf_counter() - t0
+    L.dgq_paged_info(q, C.byref(nl), C.byref(K), st)
+    stats = list(st)
+    q64.seed([seed])
+    got = 0
+    t1 = time.perf_counter()
+    for i in range(n_data):
+        if q64.measure(q, i):
+            got |= 1 << i
+    measure_s = time.perf_counter() - t1
+    q64.destroyQureg(q, env)
+    sections, remaps, run, skipped, copy_ms, hidden_ms, comp_ms, wall_ms, b_in, b_out = stats
+    return {
+        "workload": f"{n_data + 1}-qubit Bernstein-Vazirani (seed {seed}), FP64 "
+                    f"({16 << (n_data + 1) >> 30} GiB in pinned host memory), device budget {budget_bytes >> 30} GiB",
+        "pieces": 1 << K.value, "piece_qubits": nl.value,
+        "recovered_hidden_string": got == hidden,
+        "flush_s": round(flush_s, 3), "measure_all_s": round(measure_s, 3),
+        "sections": int(sections), "qubit_moves": int(remaps),
+        "pieces_run": int(run), "pieces_skipped_zero": int(skipped),
+        "copy_ms": round(copy_ms, 1), "compute_ms": round(comp_ms, 1),
+        "overlap_fraction": round(hidden_ms / copy_ms, 4) if copy_ms > 0 else None,
+        "link_GBps": round((b_in + b_out) / (wall_ms * 1e-3) / 1e9, 1) if wall_ms > 0 else None,
     }
 
 
@@ -214,6 +271,8 @@ def main():
             "avg_launch_ms": round(gpu_ms / (args.steps * max(1, launches)), 4),
         },
     }
+    if args.paged_bv:
+        out["paged_bv"] = paged_bv(q64, env, args.paged_bv - 1, int(args.paged_budget_gib * (1 << 30)))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_seconds)
     print(json.dumps(out), flush=True)

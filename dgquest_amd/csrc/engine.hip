@@ -153,106 +153,178 @@ constexpr int sweep_waves_per_eu() {
     return w > 8 ? 8 : (w < 1 ? 1 : w);
 }
 
-template <typename T, int R, int W, bool SPLIT>
-__global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R>()))
-sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
-             const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
-             const uint32_t *__restrict__ pprog) {
-    using V = typename V2<T>::type;
-    using GV = __attribute__((address_space(1))) V;
-    constexpr int NR = 1 << R;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-
-    const dq_job job = jobs[blockIdx.y];
-    const dq_block *__restrict__ blk = blocks + job.block;
-    GV *__restrict__ st = (GV *)(job.state);
-    const uint32_t tid = threadIdx.x;
-
-    // non-tile qubits come from the workgroup index
+// Non-tile qubits of tile number `t`: its bits deposited into nontile_mask.
+__device__ __forceinline__ uint64_t tile_base(uint64_t msk, uint64_t t) {
     uint64_t base = 0;
-    {
-        uint64_t msk = blk->nontile_mask;
-        uint64_t b = blockIdx.x;
-        while (msk) {
-            const int p = __builtin_ctzll(msk);
-            base |= (b & 1ull) << p;
-            b >>= 1;
-            msk &= msk - 1;
-        }
+    while (msk) {
+        const int p = __builtin_ctzll(msk);
+        base |= (t & 1ull) << p;
+        t >>= 1;
+        msk &= msk - 1;
     }
+    return base;
+}
 
-    const dq_phase *__restrict__ ph = phases + blk->phase_begin;
-    const uint32_t nph = blk->phase_count;
-    Regs<T, R> a;
-
-    // The wave index is uniform across a wavefront: with it in a scalar
-    // register the whole address except the lane part is scalar, and every
-    // load and store is "scalar base + lane * sizeof(V)" with no vector math.
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t lane = tid & 63u;
-    {   // load: lane l = qubit l, register / wave bits = ld_reg / ld_wave.
-        uint64_t wb = 0;
+// Load a tile into registers: lane l = qubit l, register / wave bits =
+// ld_reg / ld_wave.  The wave index wv is wave-uniform (a scalar), so every
+// address is "scalar base + lane * sizeof(V)" with no vector math.
+template <typename T, int R, int W>
+__device__ __forceinline__ void load_tile(Regs<T, R> &a, const dq_block *__restrict__ blk,
+                                          const __attribute__((address_space(1))) typename V2<T>::type *st,
+                                          uint64_t base, uint32_t wv, uint32_t lane) {
+    using V = typename V2<T>::type;
+    constexpr int NR = 1 << R;
+    uint64_t wb = 0;
 #pragma unroll
-        for (int w = 0; w < W; w++) wb |= ((wv >> w) & 1u) ? blk->ld_wave[w] : 0ull;
-        uint64_t qo[R > 0 ? R : 1];
+    for (int w = 0; w < W; w++) wb |= ((wv >> w) & 1u) ? blk->ld_wave[w] : 0ull;
+    uint64_t qo[R > 0 ? R : 1];
 #pragma unroll
-        for (int r = 0; r < R; r++) qo[r] = blk->ld_reg[r];
-        const uint32_t ik = blk->init_kind;
-        if (ik == DQ_INIT_LOAD) {
-#pragma unroll
-            for (int j = 0; j < NR; j++) {
-                uint64_t u = base | wb;
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    if ((j >> r) & 1) u |= qo[r];
-                const GV *p = st + u;
-                const V v = __builtin_nontemporal_load(p + lane);
-                a.re[j] = v.x;
-                a.im[j] = v.y;
-            }
-        } else if (ik == DQ_INIT_CLASSICAL) {
-            const uint64_t want = blk->init_index;
-#pragma unroll
-            for (int j = 0; j < NR; j++) {
-                uint64_t u = base | wb;
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    if ((j >> r) & 1) u |= qo[r];
-                a.re[j] = ((u | lane) == want) ? T(1) : T(0);
-                a.im[j] = T(0);
-            }
-        } else {
-            const T v = (T)blk->init_value;
-#pragma unroll
-            for (int j = 0; j < NR; j++) { a.re[j] = v; a.im[j] = T(0); }
-        }
-    }
-
-    const uint32_t *__restrict__ pp = pprog + 2 * blk->phase_begin;
-    for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W, SPLIT, T>(a, (void *)lds_raw, ph + (p - 1), ph + p, p > 1, tid);
-        AsmOps<T, R>::run(a, prog + pp[2 * p], pp[2 * p + 1], base, tid);
-    }
-
-    {   // store: lane l = qubit l, register / wave bits = st_reg / st_wave
-        uint64_t wb = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) wb |= ((wv >> w) & 1u) ? blk->st_wave[w] : 0ull;
-        uint64_t qo[R > 0 ? R : 1];
-#pragma unroll
-        for (int r = 0; r < R; r++) qo[r] = blk->st_reg[r];
+    for (int r = 0; r < R; r++) qo[r] = blk->ld_reg[r];
+    const uint32_t ik = blk->init_kind;
+    if (ik == DQ_INIT_LOAD) {
 #pragma unroll
         for (int j = 0; j < NR; j++) {
             uint64_t u = base | wb;
 #pragma unroll
             for (int r = 0; r < R; r++)
                 if ((j >> r) & 1) u |= qo[r];
-            GV *p = st + u;
-            V v;
-            v.x = a.re[j];
-            v.y = a.im[j];
-            __builtin_nontemporal_store(v, p + lane);
+            const V v = __builtin_nontemporal_load(st + u + lane);
+            a.re[j] = v.x;
+            a.im[j] = v.y;
         }
+    } else if (ik == DQ_INIT_CLASSICAL) {
+        const uint64_t want = blk->init_index;
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            uint64_t u = base | wb;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if ((j >> r) & 1) u |= qo[r];
+            a.re[j] = ((u | lane) == want) ? T(1) : T(0);
+            a.im[j] = T(0);
+        }
+    } else {
+        const T v = (T)blk->init_value;
+#pragma unroll
+        for (int j = 0; j < NR; j++) { a.re[j] = v; a.im[j] = T(0); }
+    }
+}
+
+// Store a tile: lane l = qubit l, register / wave bits = st_reg / st_wave.
+// (A macro expanded in the kernel body: as a function taking the tile by
+// reference, the FP32 instantiations kept part of the tile in scratch.)
+#define DQ_STORE_TILE(A, BASE)                                                            \
+    do {                                                                                  \
+        uint64_t wb_ = 0;                                                                 \
+        _Pragma("unroll") for (int w = 0; w < W; w++) wb_ |= ((wv >> w) & 1u) ? blk->st_wave[w] : 0ull; \
+        uint64_t qo_[R > 0 ? R : 1];                                                      \
+        _Pragma("unroll") for (int r = 0; r < R; r++) qo_[r] = blk->st_reg[r];            \
+        _Pragma("unroll") for (int j = 0; j < (1 << R); j++) {                            \
+            uint64_t u = (BASE) | wb_;                                                    \
+            _Pragma("unroll") for (int r = 0; r < R; r++) if ((j >> r) & 1) u |= qo_[r];  \
+            GV *p_ = st + u;                                                              \
+            typename V2<T>::type v_;                                                      \
+            v_.x = (A).re[j];                                                             \
+            v_.y = (A).im[j];                                                             \
+            __builtin_nontemporal_store(v_, p_ + lane);                                   \
+        }                                                                                 \
+    } while (0)
+
+// The phases of a block on one tile: exchanges through LDS between phases,
+// the op program of each phase in the assembly interpreter.  lead: a barrier
+// before the first exchange (LDS may still be read for a previous tile).
+template <typename T, int R, int W, bool SPLIT>
+__device__ __forceinline__ void process_tile(Regs<T, R> &a, void *lds, const dq_phase *__restrict__ ph, uint32_t nph,
+                                             const uint32_t *__restrict__ pp, const unsigned char *__restrict__ prog,
+                                             uint64_t cbase, uint32_t tid, bool lead) {
+    for (uint32_t p = 0; p < nph; p++) {
+        if (p > 0) exchange<R, W, SPLIT, T>(a, lds, ph + (p - 1), ph + p, p > 1 || lead, tid);
+        AsmOps<T, R>::run(a, prog + pp[2 * p], pp[2 * p + 1], cbase, tid);
+    }
+}
+
+template <typename T, int R, int W, bool SPLIT>
+__global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R>()))
+sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
+             const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
+             const uint32_t *__restrict__ pprog) {
+    using GV = __attribute__((address_space(1))) typename V2<T>::type;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const dq_job job = jobs[blockIdx.y];
+    const dq_block *__restrict__ blk = blocks + job.block;
+    GV *__restrict__ st = (GV *)(job.state);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t lane = tid & 63u;
+    const uint64_t base = tile_base(blk->nontile_mask, blockIdx.x);
+    Regs<T, R> a;
+    load_tile<T, R, W>(a, blk, st, base, wv, lane);
+    process_tile<T, R, W, SPLIT>(a, lds_raw, phases + blk->phase_begin, blk->phase_count,
+                                 pprog + 2 * blk->phase_begin, prog, base | job.base_hi, tid, false);
+    DQ_STORE_TILE(a, base);
+}
+
+// Occupancy target of the pipelined form: two register tiles live at once.
+template <typename T, int R>
+constexpr int pipe_waves_per_eu() {
+    constexpr int amp = (1 << R) * (int)(2 * sizeof(T) / 4);
+    constexpr int budget = 2 * amp + 40;
+    constexpr int w = 512 / budget;
+    return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+// Persistent, software-pipelined sweep of one state: workgroup g runs tiles
+// g, g + G, g + 2G, ... (G = gridDim.x) and issues the loads of its next tile
+// into a second register tile before it computes the current one, so each
+// wave keeps reads in flight while its VALU, LDS and scalar work runs.  The
+// interpreter's asm statement is a compiler barrier for memory operations,
+// which keeps the prefetch loads where they are issued.
+template <typename T, int R, int W, bool SPLIT>
+__global__ void __launch_bounds__(64 << W, (pipe_waves_per_eu<T, R>()))
+sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
+                  const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
+                  const uint32_t *__restrict__ pprog) {
+    using GV = __attribute__((address_space(1))) typename V2<T>::type;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const dq_job job = jobs[blockIdx.y];
+    const dq_block *__restrict__ blk = blocks + job.block;
+    GV *__restrict__ st = (GV *)(job.state);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t lane = tid & 63u;
+    const uint64_t msk = blk->nontile_mask;
+    const uint64_t ntiles = 1ull << __builtin_popcountll(msk);
+    const uint64_t G = gridDim.x;
+    const dq_phase *__restrict__ ph = phases + blk->phase_begin;
+    const uint32_t nph = blk->phase_count;
+    const uint32_t *__restrict__ pp = pprog + 2 * blk->phase_begin;
+    const uint64_t hi = job.base_hi;
+    uint64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    Regs<T, R> a, b;
+    uint64_t ba = tile_base(msk, t);
+    load_tile<T, R, W>(a, blk, st, ba, wv, lane);
+    bool lead = false;
+    while (true) {
+        const uint64_t t1 = t + G;
+        uint64_t bb = 0;
+        if (t1 < ntiles) {
+            bb = tile_base(msk, t1);
+            load_tile<T, R, W>(b, blk, st, bb, wv, lane);
+        }
+        process_tile<T, R, W, SPLIT>(a, lds_raw, ph, nph, pp, prog, ba | hi, tid, lead);
+        DQ_STORE_TILE(a, ba);
+        lead = true;
+        if (t1 >= ntiles) break;
+        const uint64_t t2 = t1 + G;
+        if (t2 < ntiles) {
+            ba = tile_base(msk, t2);
+            load_tile<T, R, W>(a, blk, st, ba, wv, lane);
+        }
+        process_tile<T, R, W, SPLIT>(b, lds_raw, ph, nph, pp, prog, bb | hi, tid, lead);
+        DQ_STORE_TILE(b, bb);
+        if (t2 >= ntiles) break;
+        t = t2;
     }
 }
 
@@ -282,14 +354,14 @@ small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
         for (uint32_t k = 0; k < ph[p].op_count; k++) {
             const dq_op *__restrict__ op = ops + ph[p].op_begin + k;
             const T *__restrict__ m = reinterpret_cast<const T *>(op->m);
-            const uint64_t c = op->c_wg, v = op->v_wg;
+            const uint64_t c = op->c_wg, v = op->v_wg, hi = job.base_hi;
             __syncthreads();
             if (op->kind == DQ_OP_MAT || op->kind == DQ_OP_SWAP || op->kind == DQ_OP_RMAT || op->kind == DQ_OP_SHEAR) {
                 const uint32_t t = op->tbit, tm = 1u << t;
                 for (uint32_t k2 = tid; k2 < N / 2; k2 += blockDim.x) {
                     const uint32_t i0 = ((k2 >> t) << (t + 1)) | (k2 & (tm - 1));
                     const uint32_t i1 = i0 | tm;
-                    if ((i0 & c) != v) continue;
+                    if (((i0 | hi) & c) != v) continue;
                     const V x = s[i0], y = s[i1];
                     if (op->kind == DQ_OP_SWAP) { s[i0] = y; s[i1] = x; continue; }
                     if (op->kind == DQ_OP_SHEAR) {
@@ -315,7 +387,7 @@ small_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
                 }
             } else {
                 for (uint32_t i = tid; i < N; i += blockDim.x) {
-                    if ((i & c) != v) continue;
+                    if (((i | hi) & c) != v) continue;
                     V x = s[i];
                     if (op->kind == DQ_OP_NEG) { x.x = -x.x; x.y = -x.y; s[i] = x; continue; }
                     V y;
@@ -454,11 +526,15 @@ typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, con
 template <typename T>
 struct SweepTable {
     sweep_fn fn[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
+    sweep_fn pipe[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
     SweepTable() {
         memset(fn, 0, sizeof fn);
+        memset(pipe, 0, sizeof pipe);
 #define DQ_INST(R_, W_)                                                                  \
     fn[0][R_][W_] = sweep_kernel<T, R_, W_, false>;                                      \
-    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;
+    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;                                       \
+    pipe[0][R_][W_] = sweep_pipe_kernel<T, R_, W_, false>;                               \
+    pipe[1][R_][W_] = sweep_pipe_kernel<T, R_, W_, true>;
         DQ_INST(1, 0) DQ_INST(2, 0) DQ_INST(3, 0) DQ_INST(4, 0) DQ_INST(5, 0)
         DQ_INST(2, 1) DQ_INST(3, 1) DQ_INST(4, 1) DQ_INST(5, 1)
         DQ_INST(2, 2) DQ_INST(3, 2) DQ_INST(4, 2) DQ_INST(5, 2)
@@ -480,6 +556,21 @@ static bool use_split(int prec, int T) {
     if (g_split_mode == 0) return false;
     if (g_split_mode == 1) return true;
     return prec == DQ_F64 && T >= 12;
+}
+
+// persistent pipelined sweeps for single-state launches: on unless DGQ_PIPE=0
+static int g_pipe_mode = -1;
+static bool use_pipe() {
+    if (g_pipe_mode < 0) {
+        const char *e = getenv("DGQ_PIPE");
+        g_pipe_mode = e ? atoi(e) : 1;
+    }
+    return g_pipe_mode != 0;
+}
+
+static sweep_fn lookup_pipe(int prec, int R, int W, bool split) {
+    if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS) return nullptr;
+    return prec == DQ_F64 ? g_tab64.pipe[split][R][W] : g_tab32.pipe[split][R][W];
 }
 
 static sweep_fn lookup(int prec, int R, int W, bool split = false) {
@@ -537,6 +628,10 @@ extern "C" int dq_device_init(int device) {
                         HIPC(hipFuncSetAttribute((const void *)g_tab64.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
                     if (g_tab32.fn[sp][R][W])
                         HIPC(hipFuncSetAttribute((const void *)g_tab32.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                    if (g_tab64.pipe[sp][R][W])
+                        HIPC(hipFuncSetAttribute((const void *)g_tab64.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                    if (g_tab32.pipe[sp][R][W])
+                        HIPC(hipFuncSetAttribute((const void *)g_tab32.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
                 }
         {
             hipDeviceProp_t prop;
@@ -789,6 +884,9 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
     }
     for (uint32_t j = 0; j < n_jobs; j++)
         if (jobs[j].block >= n_blocks || !jobs[j].state) return fail(DQ_ERR_ARG, "job %u: bad block or state", j);
+    for (uint32_t j = 0; j < n_jobs; j++)
+        if (jobs[j].base_hi & ((1ull << blocks[jobs[j].block].n_qubits) - 1))
+            return fail(DQ_ERR_ARG, "job %u: base_hi overlaps the piece's own qubits", j);
     for (uint32_t l = 0; l < n_launch; l++) {
         if (job_begin[l + 1] <= job_begin[l]) return fail(DQ_ERR_ARG, "launch %u: empty job range", l);
         const dq_block &b0 = blocks[jobs[job_begin[l]].block];
@@ -860,9 +958,21 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             if (lds > 131072) return fail(DQ_ERR_UNSUPPORTED, "tile of %d bits needs %zu bytes of LDS", b0.tile_bits, lds);
             const uint64_t tiles = 1ull << (n - b0.tile_bits);
             if (tiles > 0x7fffffffull || nj > 65535) return fail(DQ_ERR_UNSUPPORTED, "grid too large");
-            sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
-            hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                               d_jobs + j0, d_blocks, d_phases, d_prog, d_pprog);
+            // (R = 5 keeps the one-pass form: two 32-amplitude tiles do not fit the registers)
+            sweep_fn pf = (nj == 1 && b0.reg_bits <= 4 && use_pipe()) ? lookup_pipe(prec, b0.reg_bits, b0.wave_bits, split)
+                                                                     : nullptr;
+            int per_cu = 0;
+            if (pf) HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pf, 64 << b0.wave_bits, lds));
+            const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 0) * (uint64_t)g_num_cus;
+            if (pf && resident > 0 && tiles >= 2 * resident) {
+                // persistent: every resident workgroup walks its share of the tiles
+                hipLaunchKernelGGL(pf, dim3((unsigned)resident, 1), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
+                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_pprog);
+            } else {
+                sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
+                hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
+                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_pprog);
+            }
         }
         HIPC(hipGetLastError());
     }

@@ -214,6 +214,13 @@ Dag Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t
     return next;
 }
 
+uint64_t Dag::front_targets() const {
+    uint64_t m = 0;
+    for (int f : front_)
+        if (nodes_[f].alive) m |= gate_target_bits(nodes_[f].g);
+    return m;
+}
+
 std::vector<Gate> Dag::gates() const {
     std::vector<Gate> v;
     v.reserve(alive_count_);

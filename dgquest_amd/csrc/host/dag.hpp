@@ -36,6 +36,9 @@ public:
     // All surviving gates in insertion order (a valid topological order).
     std::vector<Gate> gates() const;
 
+    // Non-diagonal targets of the gates that have no predecessor.
+    uint64_t front_targets() const;
+
 private:
     struct Node {
         Gate g;

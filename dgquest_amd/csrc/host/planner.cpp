@@ -573,13 +573,14 @@ void set_end_masks(dq_block &blk, const Layout &first, const Layout &last, const
     }
 }
 
-void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, const TuneParams &t, int prec, int init_kind,
-                uint64_t init_index, double init_value, Fuse &F, bool final, Plan &P) {
+void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, const TuneParams &t, int prec,
+                int init_kind, uint64_t init_index, double init_value, Fuse &F, bool final, Plan &P) {
     const int T = 6 + t.R + t.W, R = t.R, W = t.W;
     // tile qubits: the block's targets (always containing 0..5), padded with
-    // the lowest other qubits (src/cpp/QuEST_gpu_kernel.cu:505-510 pads low to high)
+    // the lowest other qubits (src/cpp/QuEST_gpu_kernel.cu:505-510 pads low to
+    // high); all below nl (the piece's own qubits)
     uint64_t tile = tmask | 0x3Full;
-    for (int q = 6; q < n && popc64(tile) < T; q++) tile |= 1ull << q;
+    for (int q = 6; q < nl && popc64(tile) < T; q++) tile |= 1ull << q;
     TileCtx c;
     c.n = n; c.T = T; c.R = R; c.W = W; c.prec = prec;
     for (int q = 0; q < 64; q++) c.q2t[q] = -1;
@@ -591,14 +592,14 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, const TunePa
         for (int q = 0; q < n; q++)
             if ((tile >> q) & 1) { tile_q[b] = (uint8_t)q; c.q2t[q] = b; b++; }
     }
-    blk.n_qubits = (uint32_t)n;
+    blk.n_qubits = (uint32_t)nl;
     blk.tile_bits = (uint32_t)T;
     blk.reg_bits = (uint32_t)R;
     blk.wave_bits = (uint32_t)W;
     blk.init_kind = (uint32_t)init_kind;
     blk.init_index = init_index;
     blk.init_value = init_value;
-    blk.nontile_mask = ((1ull << n) - 1) & ~tile;
+    blk.nontile_mask = ((1ull << nl) - 1) & ~tile;
 
     // split the block into register phases of at most R targets each
     std::vector<std::vector<Gate>> sets;
@@ -651,14 +652,15 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, const TunePa
 }  // namespace
 
 void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind, uint64_t init_index,
-                  double init_value, const TuneParams &tp, Plan &P) {
+                  double init_value, const TuneParams &tp, Plan &P, int n_local) {
+    if (n_local < 0 || n_local > n) n_local = n;
     P.blocks.clear();
     P.phases.clear();
     P.ops.clear();
     P.st = PlanStats();
     P.st.gates_in = (int)gates.size();
 
-    if (n <= kSmallMax) {
+    if (n_local <= kSmallMax) {
         Dag d(n, true);
         for (const auto &g : gates) d.add(g);
         const std::vector<Gate> gl = d.gates();
@@ -668,8 +670,8 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         c.prec = prec;
         dq_block blk;
         memset(&blk, 0, sizeof blk);
-        blk.n_qubits = (uint32_t)n;
-        blk.tile_bits = (uint32_t)n;
+        blk.n_qubits = (uint32_t)n_local;
+        blk.tile_bits = (uint32_t)n_local;
         blk.init_kind = (uint32_t)init_kind;
         blk.init_index = init_index;
         blk.init_value = init_value;
@@ -691,7 +693,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         return;
     }
 
-    const TuneParams t = fit_shape(tp, n);
+    const TuneParams t = fit_shape(tp, n_local);
     Dag d(n, true);
     for (const auto &g : gates) d.add(g);
     Fuse F(n);
@@ -713,7 +715,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         for (const Gate &g : bg)
             if (!gate_is_diagonal(g.type)) F.left[ctz64(g.tmask)]++;
     for (size_t i = 0; i < bgs.size(); i++)
-        emit_block(bgs[i], tms[i], n, t, prec, i == 0 ? init_kind : DQ_INIT_LOAD, init_index, init_value, F,
+        emit_block(bgs[i], tms[i], n, n_local, t, prec, i == 0 ? init_kind : DQ_INIT_LOAD, init_index, init_value, F,
                    i + 1 == bgs.size(), P);
     P.st.ops = (int)P.ops.size();
 }

@@ -50,8 +50,13 @@ struct Plan {
 // Plan `gates` (physical qubit indices) on an n-qubit state.  The first block
 // generates the state instead of reading it when init_kind is
 // DQ_INIT_CLASSICAL / DQ_INIT_PLUS.
+//
+// n_local < n plans for a register held in pieces of 2^n_local amplitudes
+// (the qubits n_local .. n-1 index the piece): every non-diagonal target must
+// be below n_local; controls and diagonals on the piece qubits become
+// workgroup conditions, resolved per piece through dq_job.base_hi.
 void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind, uint64_t init_index,
-                  double init_value, const TuneParams &tp, Plan &out);
+                  double init_value, const TuneParams &tp, Plan &out, int n_local = -1);
 
 // A pass that only moves qubits: afterwards physical qubit dst[i] holds what
 // physical qubit src[i] held.  All moved qubits must fit one tile.

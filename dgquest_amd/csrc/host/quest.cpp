@@ -23,6 +23,7 @@
 
 #include "QuEST.h"
 #include "dag.hpp"
+#include "paging.hpp"
 #include "runtime.hpp"
 
 using namespace dgq;
@@ -233,6 +234,7 @@ static void run_plan(QuregState *s, const Plan &P) {
         jobs[i].state = s->d_state;
         jobs[i].block = i;
         jobs[i].pad = 0;
+        jobs[i].base_hi = 0;
         begin[i] = i;
     }
     begin[nb] = nb;
@@ -245,6 +247,17 @@ static void run_plan(QuregState *s, const Plan &P) {
 void flush(QuregState *s) {
     if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
     if (s->pending.empty() && s->init_kind == DQ_INIT_LOAD) return;
+    if (s->paged) {
+        const size_t nrec = s->pending.size();
+        paged_flush(s);
+        s->last_stats = PlanStats();
+        s->last_stats.gates_in = (int)nrec;
+        s->last_stats.sweeps = s->paged->st.sections;
+        s->last_bytes = s->paged->st.bytes_in + s->paged->st.bytes_out;
+        s->last_launches = s->paged->st.sections;
+        s->last_flush_ms = s->paged->st.wall_ms;
+        return;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     const unsigned long long state_bytes = (unsigned long long)(s->prec == DQ_F64 ? 16 : 8) << s->n;
     if (s->pending.empty()) {
@@ -270,6 +283,10 @@ void flush(QuregState *s) {
 void ensure_probs(QuregState *s) {
     flush(s);
     if (!s->probs_dirty) return;
+    if (s->paged) {
+        paged_probs(s);
+        return;
+    }
     ensure_ws(s, dq_prob_ws_bytes(s->n));
     s->p0.assign(s->n + 1, 0.0);
     ENG(dq_prob_zero_all(s->d_state, s->n, s->prec, s->p0.data(), s->d_ws, s->ws_cap, s->stream));
@@ -421,8 +438,14 @@ Qureg createQureg(int numQubits, QuESTEnv env) {
     s->p2l.resize(numQubits);
     for (int i = 0; i < numQubits; i++) s->l2p[i] = s->p2l[i] = i;
     const size_t bytes = (size_t)(sizeof(qreal) * 2) << numQubits;
-    int rc = dq_malloc(&s->d_state, bytes);
-    if (rc != DQ_OK) engine_fatal("createQureg: state allocation", rc);
+    const size_t budget = device_budget();
+    if (bytes > budget) {
+        // larger than the device may hold: pinned host pieces paged through it
+        s->paged = paged_create(numQubits, s->prec, budget);
+    } else {
+        int rc = dq_malloc(&s->d_state, bytes);
+        if (rc != DQ_OK) engine_fatal("createQureg: state allocation", rc);
+    }
     ENG(dq_stream_create(&s->stream));
     q.state = s;
     initZeroState(q);
@@ -438,7 +461,8 @@ void destroyQureg(Qureg qureg, QuESTEnv env) {
         return;
     }
     dq_stream_sync(s->stream);
-    dq_free(s->d_state);
+    if (s->paged) paged_destroy(s->paged);
+    else dq_free(s->d_state);
     if (s->d_ws) dq_free(s->d_ws);
     dq_stream_destroy(s->stream);
     delete s;
@@ -494,6 +518,10 @@ void initStateFromAmps(Qureg qureg, qreal *reals, qreal *imags) {
         const uint64_t c = (N - off < chunk) ? N - off : chunk;
         re.assign(reals + off, reals + off + c);
         im.assign(imags + off, imags + off + c);
+        if (s->paged) {
+            paged_write(s, off, c, re.data(), im.data());
+            continue;
+        }
         ensure_ws(s, 2 * ((c * sizeof(double) + 255) / 256 * 256));
         ENG(dq_write_amps(s->d_state, s->n, s->prec, off, c, re.data(), im.data(), s->d_ws, s->ws_cap, s->stream));
     }
@@ -503,13 +531,18 @@ void setAmps(Qureg qureg, long long int startInd, qreal *reals, qreal *imags, lo
     v_statevec(qureg, __func__);
     v_num_amps(qureg, startInd, numAmps, __func__);
     QuregState *s = st(qureg);
-    materialize_mapping(s);  // flushes; afterwards logical index == physical index
+    if (s->paged) flush(s);          // host pieces are written through the qubit map
+    else materialize_mapping(s);     // flushes; afterwards logical index == physical index
     const uint64_t chunk = 1ull << 22;
     std::vector<double> re, im;
     for (uint64_t off = 0; off < (uint64_t)numAmps; off += chunk) {
         const uint64_t c = ((uint64_t)numAmps - off < chunk) ? (uint64_t)numAmps - off : chunk;
         re.assign(reals + off, reals + off + c);
         im.assign(imags + off, imags + off + c);
+        if (s->paged) {
+            paged_write(s, (uint64_t)startInd + off, c, re.data(), im.data());
+            continue;
+        }
         ensure_ws(s, 2 * ((c * sizeof(double) + 255) / 256 * 256));
         ENG(dq_write_amps(s->d_state, s->n, s->prec, (uint64_t)startInd + off, c, re.data(), im.data(), s->d_ws,
                           s->ws_cap, s->stream));
@@ -711,7 +744,8 @@ Complex getAmp(Qureg qureg, long long int index) {
     flush(s);
     ensure_ws(s, 4096);
     double out[2];
-    ENG(dq_read_amps(s->d_state, s->n, s->prec, s->l2p.data(), (uint64_t)index, 1, out, s->d_ws, s->ws_cap, s->stream));
+    if (s->paged) paged_read(s, (uint64_t)index, 1, out);
+    else ENG(dq_read_amps(s->d_state, s->n, s->prec, s->l2p.data(), (uint64_t)index, 1, out, s->d_ws, s->ws_cap, s->stream));
     Complex c;
     c.real = (qreal)out[0];
     c.imag = (qreal)out[1];
@@ -851,6 +885,10 @@ int dgq_read_state(Qureg qureg, unsigned long long start, unsigned long long cou
     const uint64_t N = 1ull << s->n;
     if (start > N || count > N - start) return -1;
     flush(s);
+    if (s->paged) {
+        paged_read(s, start, count, out);
+        return 0;
+    }
     const uint64_t chunk = 1ull << 22;
     for (uint64_t off = 0; off < count; off += chunk) {
         const uint64_t c = (count - off < chunk) ? count - off : chunk;
@@ -936,6 +974,22 @@ int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_bl
     if (counts[0]) memcpy(blocks, P.blocks.data(), P.blocks.size() * sizeof(dq_block));
     if (counts[1]) memcpy(phases, P.phases.data(), P.phases.size() * sizeof(dq_phase));
     if (counts[2]) memcpy(ops, P.ops.data(), P.ops.size() * sizeof(dq_op));
+    return 0;
+}
+
+int dgq_paged_info(Qureg q, int *piece_qubits, int *pieces_log2, double stats[10]) {
+    QuregState *s = q.state;
+    if (!s) return -1;
+    if (!s->paged) return 1;
+    const PagedStore &P = *s->paged;
+    if (piece_qubits) *piece_qubits = P.nl;
+    if (pieces_log2) *pieces_log2 = P.K;
+    if (stats) {
+        const PagedStats &t = P.st;
+        const double v[10] = {(double)t.sections, (double)t.remaps, (double)t.pieces_run, (double)t.pieces_skipped,
+                              t.copy_ms, t.hidden_ms, t.compute_ms, t.wall_ms, (double)t.bytes_in, (double)t.bytes_out};
+        for (int i = 0; i < 10; i++) stats[i] = v[i];
+    }
     return 0;
 }
 

@@ -14,6 +14,10 @@
 #include "gates.hpp"
 #include "planner.hpp"
 
+namespace dgq {
+struct PagedStore;
+}
+
 struct QuESTEnvValue {
     int device = 0;
     dgq::TuneParams tune;       // default fusion shape for new registers
@@ -40,6 +44,7 @@ struct QuregState {
     int last_launches = 0;
     double last_flush_ms = 0.0;
     bool recorder = false;            // records and plans only, no device state
+    dgq::PagedStore *paged = nullptr; // state in pinned host pieces, paged through the device
 };
 
 namespace dgq {

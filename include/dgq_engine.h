@@ -138,6 +138,13 @@ typedef struct dq_job {
     void *state;
     uint32_t block; /* index into the block array of the call */
     uint32_t pad;
+    /* Index bits of this state piece inside a larger register (bits at and
+     * above the block's n_qubits; 0 for a whole state).  They take part in
+     * the ops' workgroup conditions (c_wg / v_wg), never in addresses: a
+     * register paged through the device in pieces runs one plan on every
+     * piece, controls and diagonals on the piece-index qubits resolved per
+     * piece. */
+    uint64_t base_hi;
 } dq_job;
 
 /* ---- runtime plumbing ------------------------------------------------------ */

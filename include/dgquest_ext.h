@@ -64,6 +64,15 @@ typedef struct dgq_gate_rec {
 int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k, int R, int W, int max_ops,
                   int stats[8]);
 
+/* Registers larger than the device budget (DGQ_HBM_BUDGET bytes when set,
+ * else the device's free memory less a margin) live in pinned host memory as
+ * 2^K pieces of 2^nl amplitudes and are paged through the device.  Returns 1
+ * for a register resident on the device, 0 for a paged one, filling nl, K and
+ * the statistics of its last flush: sections, high <-> piece qubit moves,
+ * pieces run, zero pieces skipped, copy ms, copy ms hidden under compute,
+ * compute ms, wall ms, bytes copied in, bytes copied out. */
+int dgq_paged_info(Qureg q, int *piece_qubits, int *pieces_log2, double stats[10]);
+
 /* A register that only records and plans, with no device state (no GPU
  * needed): the gate calls of include/QuEST.h validate and record on it as on
  * any register, so the front end and the planner can be checked on any host.
