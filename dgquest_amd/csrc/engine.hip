@@ -97,43 +97,49 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, void *lds_raw, const dq_
     uint32_t ra[R > 0 ? R : 1], rb[R > 0 ? R : 1];
 #pragma unroll
     for (int r = 0; r < R; r++) { ra[r] = swz(A->reg_t[r]) * ES; rb[r] = swz(B->reg_t[r]) * ES; }
-    uint32_t oa[NR], ob[NR];
-#pragma unroll
-    for (int j = 0; j < NR; j++) {
-        uint32_t xa = 0, xb = 0;
+    // register j's offset = thread part xor a uniform (scalar) part, formed at
+    // each access: one vector xor, no offset arrays held in VGPRs
+    auto offa = [&](int j) {
+        uint32_t x = 0;
 #pragma unroll
         for (int r = 0; r < R; r++)
-            if ((j >> r) & 1) { xa ^= ra[r]; xb ^= rb[r]; }
-        oa[j] = sa ^ xa;
-        ob[j] = sb ^ xb;
-    }
+            if ((j >> r) & 1) x ^= ra[r];
+        return sa ^ x;
+    };
+    auto offb = [&](int j) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if ((j >> r) & 1) x ^= rb[r];
+        return sb ^ x;
+    };
     unsigned char *lds = reinterpret_cast<unsigned char *>(lds_raw);
     if (!SPLIT) {
         if (lead_barrier) __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            *reinterpret_cast<T *>(lds + oa[j]) = a.re[j];
-            *reinterpret_cast<T *>(lds + oa[j] + sizeof(T)) = a.im[j];
+            *reinterpret_cast<T *>(lds + offa(j)) = a.re[j];
+            *reinterpret_cast<T *>(lds + offa(j) + sizeof(T)) = a.im[j];
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            a.re[j] = *reinterpret_cast<const T *>(lds + ob[j]);
-            a.im[j] = *reinterpret_cast<const T *>(lds + ob[j] + sizeof(T));
+            a.re[j] = *reinterpret_cast<const T *>(lds + offb(j));
+            a.im[j] = *reinterpret_cast<const T *>(lds + offb(j) + sizeof(T));
         }
     } else {
         if (lead_barrier) __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + oa[j]) = a.re[j];
+        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + offa(j)) = a.re[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.re[j] = *reinterpret_cast<const T *>(lds + ob[j]);
+        for (int j = 0; j < NR; j++) a.re[j] = *reinterpret_cast<const T *>(lds + offb(j));
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + oa[j]) = a.im[j];
+        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + offa(j)) = a.im[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.im[j] = *reinterpret_cast<const T *>(lds + ob[j]);
+        for (int j = 0; j < NR; j++) a.im[j] = *reinterpret_cast<const T *>(lds + offb(j));
     }
 }
 
@@ -305,22 +311,21 @@ sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ 
     uint64_t ba = tile_base(msk, t);
     load_tile<T, R, W>(a, blk, st, ba, wv, lane);
     bool lead = false;
+    // The prefetch loads are unconditional (a tile past the end is clamped to
+    // the last tile and its data left unused): a conditional load would make
+    // the compiler merge the two register tiles with copies, and a copy waits
+    // for the data, which turns the prefetch into a plain load.
     while (true) {
         const uint64_t t1 = t + G;
-        uint64_t bb = 0;
-        if (t1 < ntiles) {
-            bb = tile_base(msk, t1);
-            load_tile<T, R, W>(b, blk, st, bb, wv, lane);
-        }
+        const uint64_t bb = tile_base(msk, t1 < ntiles ? t1 : ntiles - 1);
+        load_tile<T, R, W>(b, blk, st, bb, wv, lane);
         process_tile<T, R, W, SPLIT>(a, lds_raw, ph, nph, pp, prog, ba | hi, tid, lead);
         DQ_STORE_TILE(a, ba);
         lead = true;
         if (t1 >= ntiles) break;
         const uint64_t t2 = t1 + G;
-        if (t2 < ntiles) {
-            ba = tile_base(msk, t2);
-            load_tile<T, R, W>(a, blk, st, ba, wv, lane);
-        }
+        ba = tile_base(msk, t2 < ntiles ? t2 : ntiles - 1);
+        load_tile<T, R, W>(a, blk, st, ba, wv, lane);
         process_tile<T, R, W, SPLIT>(b, lds_raw, ph, nph, pp, prog, bb | hi, tid, lead);
         DQ_STORE_TILE(b, bb);
         if (t2 >= ntiles) break;
@@ -558,12 +563,14 @@ static bool use_split(int prec, int T) {
     return prec == DQ_F64 && T >= 12;
 }
 
-// persistent pipelined sweeps for single-state launches: on unless DGQ_PIPE=0
+// persistent pipelined sweeps for single-state launches (DGQ_PIPE=1); off by
+// default: with two register tiles live the allocator copies prefetched
+// values, and each copy waits for its load (measured slower on MI355X)
 static int g_pipe_mode = -1;
 static bool use_pipe() {
     if (g_pipe_mode < 0) {
         const char *e = getenv("DGQ_PIPE");
-        g_pipe_mode = e ? atoi(e) : 1;
+        g_pipe_mode = e ? atoi(e) : 0;
     }
     return g_pipe_mode != 0;
 }
