@@ -245,7 +245,7 @@ __device__ __forceinline__ void process_tile(Regs<T, R> &a, void *lds, const dq_
                                              uint64_t cbase, uint32_t tid, bool lead) {
     for (uint32_t p = 0; p < nph; p++) {
         if (p > 0) exchange<R, W, SPLIT, T>(a, lds, ph + (p - 1), ph + p, p > 1 || lead, tid);
-        AsmOps<T, R>::run(a, prog + pp[2 * p], pp[2 * p + 1], cbase, tid);
+        AsmOps<T, R>::run(a, prog + pp[2 * p], cbase, tid);
     }
 }
 
@@ -487,6 +487,305 @@ __global__ void __launch_bounds__(256) prob_stage2(const double *__restrict__ pa
 }
 
 // ----------------------------------------------------------------------------
+// read-out of many states of one size: P(0) per qubit, total probability and
+// samples drawn by inverse transform over a tree of probability sums
+// ----------------------------------------------------------------------------
+// Level 1 of the tree holds the sums of |a|^2 over groups of 2^RO_GB
+// amplitudes; every higher level sums RO_FAN entries of the level below,
+// until a level has at most RO_FAN entries.  A sample walks down from the
+// top, reading at most RO_FAN values per level.
+#define RO_GB 10
+#define RO_FAN_BITS 10
+#define RO_FAN (1u << RO_FAN_BITS)
+#define RO_GPW 16        // level-1 groups per wave in stage 1
+#define RO_MAX_LEVELS 4  // n <= 40
+
+struct RoLevels {
+    double *base[RO_MAX_LEVELS];    // level L+1 of state s at base[L] + s * stride[L]
+    uint64_t stride[RO_MAX_LEVELS];
+    uint64_t count[RO_MAX_LEVELS];  // entries per state
+    int nlev;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Stage 1, grid (ceil(N1 / 64), count), 256 threads.  Wave w of workgroup x
+// owns level-1 groups g = 64x + 16w + k (k < 16); lane l reads amplitudes
+// g * 2^gb + 64j + l.  Index bits: 0..5 lane, 6..9 j, 10..13 k, 14..15 w,
+// 16.. workgroup -- so the 17 partial slots of a workgroup are P0 partials of
+// qubits 0..15 and its total, as in prob_stage1, and stage 2 is shared.
+template <typename T>
+__global__ void __launch_bounds__(256)
+readout_stage1(const void *const *__restrict__ states, int n, double *__restrict__ part, uint64_t part_stride,
+               double *__restrict__ lvl1, uint64_t lvl1_stride) {
+    using V = typename V2<T>::type;
+    __shared__ double red[4][PROB_SLOTS];
+    const V *__restrict__ st = reinterpret_cast<const V *>(states[blockIdx.y]);
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int gb = n < RO_GB ? n : RO_GB;
+    const uint64_t n1 = 1ull << (n - gb);
+    const uint32_t gsize = 1u << gb;
+    double tot = 0.0;
+    double zj[4] = {0.0, 0.0, 0.0, 0.0}, zk[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = 0; k < RO_GPW; k++) {
+        const uint64_t g = (uint64_t)blockIdx.x * (4 * RO_GPW) + w * RO_GPW + k;
+        if (g >= n1) break;  // uniform over the wave
+        const V *__restrict__ src = st + (g << gb);
+        double gs = 0.0;
+        if (gsize == (1u << RO_GB)) {
+            V v[16];
+#pragma unroll
+            for (uint32_t j = 0; j < 16; j++) v[j] = __builtin_nontemporal_load(src + (j << 6) + lane);
+#pragma unroll
+            for (uint32_t j = 0; j < 16; j++) {
+                const double p = (double)v[j].x * (double)v[j].x + (double)v[j].y * (double)v[j].y;
+                gs += p;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if (!((j >> b) & 1)) zj[b] += p;
+            }
+        } else {
+            const uint32_t J = gsize >= 64 ? gsize >> 6 : 1;
+            for (uint32_t j = 0; j < J; j++) {
+                const uint32_t i = (j << 6) + lane;
+                if (i >= gsize) continue;
+                const V v = src[i];
+                const double p = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                gs += p;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if (!((j >> b) & 1)) zj[b] += p;
+            }
+        }
+        tot += gs;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (!((k >> b) & 1)) zk[b] += gs;
+        const double s = wave_sum(gs);
+        if (lane == 0) lvl1[blockIdx.y * lvl1_stride + g] = s;
+    }
+    double vals[PROB_SLOTS];
+#pragma unroll
+    for (int q = 0; q < 6; q++) vals[q] = ((lane >> q) & 1) ? 0.0 : tot;
+#pragma unroll
+    for (int b = 0; b < 4; b++) vals[6 + b] = zj[b];
+#pragma unroll
+    for (int b = 0; b < 4; b++) vals[10 + b] = zk[b];
+#pragma unroll
+    for (int b = 0; b < 2; b++) vals[14 + b] = ((w >> b) & 1) ? 0.0 : tot;
+    vals[16] = tot;
+#pragma unroll
+    for (int s = 0; s < PROB_SLOTS; s++) {
+        const double r = wave_sum(vals[s]);
+        if (lane == 0) red[w][s] = r;
+    }
+    __syncthreads();
+    if (tid < PROB_SLOTS)
+        part[blockIdx.y * part_stride + (uint64_t)blockIdx.x * PROB_SLOTS + tid] =
+            (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+}
+
+// Stage 2, grid (n + 1, count): prob_stage2 per state.
+__global__ void __launch_bounds__(256) readout_stage2(const double *__restrict__ part, uint64_t part_stride,
+                                                      uint32_t ngroups, int n, double *__restrict__ out) {
+    __shared__ double red[4];
+    const int q = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const double *__restrict__ pp = part + blockIdx.y * part_stride;
+    double acc = 0.0;
+    for (uint32_t g = tid; g < ngroups; g += 256) {
+        double v;
+        if (q == n) v = pp[(uint64_t)g * PROB_SLOTS + 16];
+        else if (q < PROB_WG_BITS) v = pp[(uint64_t)g * PROB_SLOTS + q];
+        else v = ((g >> (q - PROB_WG_BITS)) & 1) ? 0.0 : pp[(uint64_t)g * PROB_SLOTS + 16];
+        acc += v;
+    }
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) out[(uint64_t)blockIdx.y * (n + 1) + q] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// One level of the tree: entry e = sum of entries RO_FAN*e .. RO_FAN*e + RO_FAN-1
+// of the level below.  Grid (ceil(out_count / 4), count), one wave per entry.
+__global__ void __launch_bounds__(256) readout_level(const double *__restrict__ in, uint64_t in_count, uint64_t in_stride,
+                                                     double *__restrict__ out, uint64_t out_count, uint64_t out_stride) {
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t e = (uint64_t)blockIdx.x * 4 + w;
+    if (e >= out_count) return;  // uniform over the wave; no barrier follows
+    const double *__restrict__ src = in + blockIdx.y * in_stride + (e << RO_FAN_BITS);
+    const uint64_t lim = in_count - (e << RO_FAN_BITS);
+    double s = 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < RO_FAN / 64; j++) {
+        const uint64_t i = (j << 6) + lane;
+        if (i < lim) s += src[i];
+    }
+    s = wave_sum(s);
+    if (lane == 0) out[blockIdx.y * out_stride + e] = s;
+}
+
+// Wave-wide pick inside a chunk of len <= 1024 values (lane l holds values
+// 16l .. 16l+15): the first index whose running sum exceeds t.  On return t
+// holds the remainder inside the picked value.  A target that rounding puts
+// at or past the chunk's total picks the last nonzero value.
+template <typename F>
+__device__ __forceinline__ uint32_t wave_pick(F value, uint32_t len, double &t, uint32_t lane) {
+    double v[16];
+    double ls = 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; j++) {
+        const uint32_t i = lane * 16 + j;
+        v[j] = i < len ? value(i) : 0.0;
+        ls += v[j];
+    }
+    double inc = ls;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(inc, o, 64);
+        if ((int)lane >= o) inc += y;
+    }
+    double exc = __shfl_up(inc, 1, 64);
+    if (lane == 0) exc = 0.0;
+    // lane-local choice: the first j past t, else the last nonzero value
+    int jsel = -1, jlast = -1;
+    double bsel = 0.0, run = exc;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const double nr = run + v[j];
+        if (jsel < 0 && nr > t) { jsel = j; bsel = run; }
+        if (v[j] > 0.0) jlast = j;
+        run = nr;
+    }
+    const uint64_t hit = __ballot(inc > t);
+    uint32_t L;
+    bool clamp;
+    if (hit) {
+        L = (uint32_t)__builtin_ctzll(hit);
+        clamp = __shfl(jsel, L, 64) < 0;
+    } else {
+        const uint64_t nz = __ballot(ls > 0.0);
+        L = nz ? 63u - (uint32_t)__builtin_clzll(nz) : 0u;
+        clamp = true;
+    }
+    const int j = clamp ? __shfl(jlast, L, 64) : __shfl(jsel, L, 64);
+    const int jj = j < 0 ? 0 : j;
+    double vsel = v[0];
+#pragma unroll
+    for (int k = 1; k < 16; k++)
+        if (k == jj) vsel = v[k];
+    vsel = __shfl(vsel, L, 64);
+    const double before = __shfl(bsel, L, 64);
+    t = clamp ? 0.5 * vsel : t - before;
+    return L * 16 + (uint32_t)jj;
+}
+
+// One wave per sample, grid (ceil(shots / 4), count).  u[s] in [0, 1); the
+// sample is the first amplitude index whose cumulative probability exceeds
+// u[s] times the total.
+template <typename T>
+__global__ void __launch_bounds__(256)
+readout_sample(const void *const *__restrict__ states, int n, RoLevels lv, const double *__restrict__ u, uint32_t shots,
+               uint64_t *__restrict__ out) {
+    using V = typename V2<T>::type;
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t s = blockIdx.x * 4 + w;
+    if (s >= shots) return;  // uniform over the wave; no barrier follows
+    const uint32_t sid = blockIdx.y;
+    const int gb = n < RO_GB ? n : RO_GB;
+    const int top = lv.nlev - 1;
+    const double *__restrict__ A = lv.base[top] + sid * lv.stride[top];
+    const uint32_t tlen = (uint32_t)lv.count[top];
+    double tl = 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; j++) {
+        const uint32_t i = lane * 16 + j;
+        if (i < tlen) tl += A[i];
+    }
+    const double total = wave_sum(tl);
+    double t = u[(uint64_t)sid * shots + s] * total;
+    uint64_t e0 = 0;  // first entry of the chunk at the current level
+    for (int L = top; L >= 0; L--) {
+        const double *__restrict__ B = lv.base[L] + sid * lv.stride[L] + e0;
+        const uint64_t rem = lv.count[L] - e0;
+        const uint32_t len = rem < RO_FAN ? (uint32_t)rem : RO_FAN;
+        const uint32_t e = wave_pick([&](uint32_t i) { return B[i]; }, len, t, lane);
+        e0 = (e0 + e) << (L > 0 ? RO_FAN_BITS : gb);
+    }
+    const V *__restrict__ st = reinterpret_cast<const V *>(states[sid]) + e0;
+    const uint32_t e = wave_pick(
+        [&](uint32_t i) {
+            const V a = st[i];
+            return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+        },
+        1u << gb, t, lane);
+    if (lane == 0) out[(uint64_t)sid * shots + s] = e0 + e;
+}
+
+// ----------------------------------------------------------------------------
+// comparison of two states of one size (any precisions): <a|b>, <a|a>, <b|b>
+// and max |a_i - b_i|
+// ----------------------------------------------------------------------------
+#define CMP_SLOTS 5
+
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(256)
+compare_stage1(const typename V2<TA>::type *__restrict__ a, const typename V2<TB>::type *__restrict__ b, int n,
+               double *__restrict__ part) {
+    __shared__ double red[4][CMP_SLOTS];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int cb = n < PROB_WG_BITS ? n : PROB_WG_BITS;
+    const uint64_t g0 = (uint64_t)blockIdx.x << cb;
+    const uint32_t cnt = 1u << cb;
+    double re = 0.0, im = 0.0, aa = 0.0, bb = 0.0, mx = 0.0;
+    for (uint32_t i = tid; i < cnt; i += 256) {
+        const typename V2<TA>::type x = a[g0 + i];
+        const typename V2<TB>::type y = b[g0 + i];
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+        re += xr * yr + xi * yi;
+        im += xr * yi - xi * yr;
+        aa += xr * xr + xi * xi;
+        bb += yr * yr + yi * yi;
+        const double dr = xr - yr, di = xi - yi;
+        mx = fmax(mx, dr * dr + di * di);
+    }
+    double v[CMP_SLOTS] = {wave_sum(re), wave_sum(im), wave_sum(aa), wave_sum(bb), wave_max(mx)};
+    if (lane == 0)
+        for (int k = 0; k < CMP_SLOTS; k++) red[w][k] = v[k];
+    __syncthreads();
+    if (tid < CMP_SLOTS) {
+        const double r = tid == 4 ? fmax(fmax(red[0][4], red[1][4]), fmax(red[2][4], red[3][4]))
+                                  : (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        part[(uint64_t)blockIdx.x * CMP_SLOTS + tid] = r;
+    }
+}
+
+// one workgroup: out = {re <a|b>, im <a|b>, <a|a>, <b|b>, max |a_i - b_i|}
+__global__ void __launch_bounds__(256) compare_stage2(const double *__restrict__ part, uint32_t ngroups, double *__restrict__ out) {
+    __shared__ double red[4][CMP_SLOTS];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    double v[CMP_SLOTS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t g = tid; g < ngroups; g += 256) {
+        for (int k = 0; k < 4; k++) v[k] += part[(uint64_t)g * CMP_SLOTS + k];
+        v[4] = fmax(v[4], part[(uint64_t)g * CMP_SLOTS + 4]);
+    }
+    for (int k = 0; k < 4; k++) v[k] = wave_sum(v[k]);
+    v[4] = wave_max(v[4]);
+    if (lane == 0)
+        for (int k = 0; k < CMP_SLOTS; k++) red[w][k] = v[k];
+    __syncthreads();
+    if (tid < CMP_SLOTS) {
+        const double r = tid == 4 ? sqrt(fmax(fmax(red[0][4], red[1][4]), fmax(red[2][4], red[3][4])))
+                                  : (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        out[tid] = r;
+    }
+}
+
+// ----------------------------------------------------------------------------
 // amplitude gather / scatter / fill
 // ----------------------------------------------------------------------------
 template <typename T>
@@ -588,7 +887,10 @@ static sweep_fn lookup(int prec, int R, int W, bool split = false) {
 static size_t elem_bytes(int prec) { return prec == DQ_F64 ? 16 : 8; }
 
 // ----------------------------------------------------------------------------
-// staging of descriptors: a ring of pinned host buffers guarded by events
+// staging of descriptors: a pool of pinned host buffers guarded by events.
+// An upload takes a buffer whose last copy has completed, so a stream with a
+// long queue never holds up uploads on other streams; the pool grows to
+// kStageMax buffers before a caller waits.
 // ----------------------------------------------------------------------------
 namespace {
 struct StageSlot {
@@ -596,8 +898,10 @@ struct StageSlot {
     size_t cap = 0;
     hipEvent_t ev = nullptr;
 };
+constexpr unsigned kStageMax = 64;
 std::mutex g_stage_mu;
-StageSlot g_stage[16];
+StageSlot g_stage[kStageMax];
+unsigned g_stage_used = 0;
 unsigned g_stage_next = 0;
 int g_device = -1;
 int g_num_cus = 256;
@@ -605,11 +909,28 @@ int g_num_cus = 256;
 
 static int stage_upload(void *d_dst, const std::vector<unsigned char> &src, void *stream) {
     std::lock_guard<std::mutex> lk(g_stage_mu);
-    StageSlot &s = g_stage[g_stage_next++ % 16];
-    if (!s.ev) HIPC(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
-    HIPC(hipEventSynchronize(s.ev));
+    StageSlot *slot = nullptr;
+    for (unsigned k = 0; k < g_stage_used && !slot; k++) {
+        StageSlot &c = g_stage[(g_stage_next + k) % g_stage_used];
+        const hipError_t e = hipEventQuery(c.ev);
+        if (e == hipSuccess) slot = &c;
+        else if (e == hipErrorNotReady) (void)hipGetLastError();
+        else HIPC(e);
+    }
+    if (!slot && g_stage_used < kStageMax) {
+        slot = &g_stage[g_stage_used++];
+        HIPC(hipEventCreateWithFlags(&slot->ev, hipEventDisableTiming));
+    }
+    if (!slot) {
+        slot = &g_stage[g_stage_next % g_stage_used];
+        HIPC(hipEventSynchronize(slot->ev));
+    }
+    g_stage_next = (unsigned)(slot - g_stage) + 1;
+    StageSlot &s = *slot;
     if (s.cap < src.size()) {
         if (s.host) HIPC(hipHostFree(s.host));
+        s.host = nullptr;
+        s.cap = 0;
         size_t cap = 64 * 1024;
         while (cap < src.size()) cap *= 2;
         HIPC(hipHostMalloc(&s.host, cap, hipHostMallocDefault));
@@ -718,83 +1039,196 @@ extern "C" int dq_stream_wait_event(void *stream, void *ev) {
 // ----------------------------------------------------------------------------
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Upper bound of the device program built from n_ops descriptors: at most
-// 96 bytes per descriptor (a complex matrix in FP64; a register table of 2^R
-// entries spans 1 + 2^R / 4 descriptors and 32 + 16 * 2^R bytes), plus
+// Upper bound of the device program built from n_ops descriptors in n_phases
+// phases: at most 128 bytes per descriptor (a complex matrix in FP64 behind a
+// workgroup-condition op; a register table of 2^R entries spans 1 + 2^R / 4
+// descriptors and at most 16 + 16 * 2^R bytes), an END op per phase, and
 // padding so the interpreter's fixed-size header loads stay in the buffer.
-static size_t prog_bound(uint32_t n_ops) { return (size_t)n_ops * 96 + 256; }
+static size_t prog_bound(uint32_t n_phases, uint32_t n_ops) { return (size_t)n_ops * 128 + (size_t)n_phases * 16 + 256; }
 
 extern "C" size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32_t n_ops, uint32_t n_jobs) {
     return align_up(n_blocks * sizeof(dq_block), 256) + align_up(n_phases * sizeof(dq_phase), 256) +
            align_up(n_ops * sizeof(dq_op), 256) + align_up(n_jobs * sizeof(dq_job), 256) +
-           align_up(n_phases * 2 * sizeof(uint32_t), 256) + align_up(prog_bound(n_ops), 256);
+           align_up(n_phases * 2 * sizeof(uint32_t), 256) + align_up(prog_bound(n_phases, n_ops), 256);
 }
 
-// Device program of one phase (format documented in gen_ops_asm.py): one
-// entry per op, a 32-byte header and the coefficients in the state precision,
-// register conditions turned into a mask over register indices.
-static void build_prog(int prec, int R, const dq_op *ops, uint32_t count, std::vector<unsigned char> &out,
-                       uint32_t &n_out) {
-    const size_t ts = prec == DQ_F64 ? 8 : 4;
-    const uint32_t NR = 1u << R;
-    n_out = 0;
-    for (uint32_t k = 0; k < count; k++) {
-        const dq_op &o = ops[k];
-        auto reg_mask = [&](bool pairs) {
-            uint32_t m = 0;
-            for (uint32_t j = 0; j < NR; j++) {
-                if (pairs && ((j >> o.tbit) & 1)) continue;
-                if ((j & o.c_reg) == o.v_reg) m |= 1u << j;
-            }
-            return m;
-        };
-        uint32_t kind = 0, bit = 0, jmask = 0;
-        size_t body = 0;
-        const unsigned char *src = reinterpret_cast<const unsigned char *>(o.m);
-        std::vector<unsigned char> table;
-        switch (o.kind) {
-            case DQ_OP_SHEAR: kind = 1; bit = o.tbit; body = 2 * ts; break;
-            case DQ_OP_DIAGR: {  // jmask: the entries that are not exactly 1
-                kind = 2;
-                table.resize(NR * 2 * ts);
-                for (uint32_t e = 0; e < NR; e++) {
-                    const unsigned char *ent = reinterpret_cast<const unsigned char *>(ops[k + 1 + e / 4].m) + (e % 4) * 2 * ts;
-                    memcpy(table.data() + e * 2 * ts, ent, 2 * ts);
-                    bool one;
-                    if (prec == DQ_F64) {
-                        double v[2];
-                        memcpy(v, ent, sizeof v);
-                        one = v[0] == 1.0 && v[1] == 0.0;
-                    } else {
-                        float v[2];
-                        memcpy(v, ent, sizeof v);
-                        one = v[0] == 1.0f && v[1] == 0.0f;
-                    }
-                    if (!one) jmask |= 1u << e;
-                }
-                src = table.data();
-                body = table.size();
-                k += o.tbit;
-                break;
-            }
-            case DQ_OP_NEG: kind = 3; jmask = reg_mask(false); break;
-            case DQ_OP_PHASE: kind = 4; jmask = reg_mask(false); body = 2 * ts; break;
-            case DQ_OP_MAT: kind = 5; bit = o.tbit; jmask = reg_mask(true); body = 8 * ts; break;
-            case DQ_OP_SWAP: kind = 6; bit = o.tbit; jmask = reg_mask(true); break;
-            case DQ_OP_RMAT: kind = 7; bit = o.tbit; body = 4 * ts; break;
-            default: break;
-        }
-        const size_t size = align_up(32 + body, 32);
-        const size_t at = out.size();
-        out.resize(at + size, 0);
-        uint32_t hdr[4] = {kind * 8 + bit, jmask, o.c_thr | (o.v_thr << 16), (uint32_t)size};
-        memcpy(out.data() + at, hdr, 16);
-        memcpy(out.data() + at + 16, &o.c_wg, 8);
-        memcpy(out.data() + at + 24, &o.v_wg, 8);
-        if (body) memcpy(out.data() + at + 32, src, body);
-        n_out++;
+// Device program of one phase (format and op codes in gen_ops_asm.py /
+// ops_asm.inc): register conditions resolved into op variants or masks,
+// consecutive unconditioned shears on distinct register bits merged into one
+// op, a WG op in front of each op with a workgroup condition, END last.
+namespace {
+struct ProgBuilder {
+    int prec, R, NR, ncodes;
+    size_t ts;
+    const DevOpCode *tab;
+    std::vector<unsigned char> &out;
+    bool bad = false;  // an op with no code (never sent to the device)
+    ProgBuilder(int prec_, int R_, std::vector<unsigned char> &o) : prec(prec_), R(R_), NR(1 << R_), out(o) {
+        ts = prec == DQ_F64 ? 8 : 4;
+        tab = dev_ops(R, ncodes);
     }
-}
+    int code(const char *name, int p0 = -1, int p1 = -1, int p2 = -1, int p3 = -1) const {
+        for (int c = 0; c < ncodes; c++)
+            if (!strcmp(tab[c].name, name) && tab[c].p[0] == p0 && tab[c].p[1] == p1 && tab[c].p[2] == p2 &&
+                tab[c].p[3] == p3)
+                return c;
+        return -1;
+    }
+    size_t size(int c) const { return (size_t)(prec == DQ_F64 ? tab[c].size64 : tab[c].size32); }
+    // one op into `v`: header then coefficient bytes, zero padded to its size
+    void encode(std::vector<unsigned char> &v, int c, uint32_t mask, uint32_t cthr, uint32_t vthr, const void *coef,
+                size_t nbytes) {
+        if (c < 0 || c >= ncodes || nbytes + 16 > size(c)) {
+            bad = true;
+            return;
+        }
+        const size_t at = v.size();
+        v.resize(at + size(c), 0);
+        const uint32_t hdr[4] = {(uint32_t)c, mask, cthr, vthr};
+        memcpy(v.data() + at, hdr, 16);
+        if (nbytes) memcpy(v.data() + at + 16, coef, nbytes);
+    }
+    void emit(const std::vector<unsigned char> &op, uint64_t c_wg, uint64_t v_wg) {
+        if (c_wg) {
+            const uint64_t cond[2] = {c_wg, v_wg};
+            encode(out, code("WG"), 0, 0, (uint32_t)op.size(), cond, 16);
+        }
+        out.insert(out.end(), op.begin(), op.end());
+    }
+    // register indices (pair ops: lower indices) satisfying (j & c) == v
+    uint32_t jmask(uint32_t c, uint32_t v, int pair_bit) const {
+        uint32_t m = 0;
+        for (int j = 0; j < NR; j++) {
+            if (pair_bit >= 0 && ((j >> pair_bit) & 1)) continue;
+            if (((uint32_t)j & c) == v) m |= 1u << j;
+        }
+        return m;
+    }
+    // NEG / PHASE variant for the register condition (c, v)
+    int cond_variant(const char *all, const char *b, const char *bb, const char *gen, uint32_t c, uint32_t v) const {
+        const int pc = __builtin_popcount(c);
+        if (pc == 0) return code(all);
+        if (pc == 1) {
+            const int r = __builtin_ctz(c);
+            return code(b, r, (int)((v >> r) & 1));
+        }
+        if (pc == 2) {
+            const int r1 = __builtin_ctz(c), r2 = 31 - __builtin_clz(c);
+            return code(bb, r1, r2, (int)((v >> r1) & 1), (int)((v >> r2) & 1));
+        }
+        return code(gen);
+    }
+    void build(const dq_op *ops, uint32_t count, uint32_t &n_out) {
+        n_out = 0;
+        std::vector<unsigned char> op;
+        for (uint32_t k = 0; k < count; k++) {
+            const dq_op &o = ops[k];
+            const unsigned char *m = reinterpret_cast<const unsigned char *>(o.m);
+            op.clear();
+            switch (o.kind) {
+                case DQ_OP_SHEAR: {
+                    // merge the run of unconditioned shears on distinct bits
+                    std::vector<unsigned char> slots(2 * (size_t)R * ts, 0);
+                    uint32_t mask = 0;
+                    uint32_t k2 = k;
+                    while (k2 < count && ops[k2].kind == DQ_OP_SHEAR && !ops[k2].c_wg && !((mask >> ops[k2].tbit) & 1)) {
+                        mask |= 1u << ops[k2].tbit;
+                        memcpy(slots.data() + 2 * ops[k2].tbit * ts, ops[k2].m, 2 * ts);
+                        k2++;
+                    }
+                    if (k2 == k) {  // a shear with a workgroup condition: on its own
+                        mask = 1u << o.tbit;
+                        memcpy(slots.data() + 2 * o.tbit * ts, o.m, 2 * ts);
+                        k2 = k + 1;
+                    }
+                    encode(op, code("SHEARS"), mask, 0, 0, slots.data(), slots.size());
+                    emit(op, k2 == k + 1 ? o.c_wg : 0, k2 == k + 1 ? o.v_wg : 0);
+                    n_out++;
+                    k = k2 - 1;
+                    continue;
+                }
+                case DQ_OP_DIAGR: {
+                    std::vector<unsigned char> table((size_t)NR * 2 * ts);
+                    uint32_t ne1 = 0;  // entries that are not exactly 1
+                    for (int e = 0; e < NR; e++) {
+                        const unsigned char *ent = reinterpret_cast<const unsigned char *>(ops[k + 1 + e / 4].m) + (e % 4) * 2 * ts;
+                        memcpy(table.data() + (size_t)e * 2 * ts, ent, 2 * ts);
+                        bool one;
+                        if (prec == DQ_F64) {
+                            double v[2];
+                            memcpy(v, ent, sizeof v);
+                            one = v[0] == 1.0 && v[1] == 0.0;
+                        } else {
+                            float v[2];
+                            memcpy(v, ent, sizeof v);
+                            one = v[0] == 1.0f && v[1] == 0.0f;
+                        }
+                        if (!one) ne1 |= 1u << e;
+                    }
+                    int half = -1;
+                    for (int r = 0; r < R && half < 0; r++)
+                        if (ne1 && (ne1 & ~jmask(1u << r, 1u << r, -1)) == 0) half = r;
+                    if (half >= 0) {
+                        std::vector<unsigned char> compact;
+                        for (int e = 0; e < NR; e++)
+                            if ((e >> half) & 1)
+                                compact.insert(compact.end(), table.begin() + (size_t)e * 2 * ts,
+                                               table.begin() + (size_t)(e + 1) * 2 * ts);
+                        encode(op, code("TABLE_HALF", half), 0, 0, 0, compact.data(), compact.size());
+                    } else if (4 * __builtin_popcount(ne1) >= 3 * NR) {
+                        encode(op, code("TABLE_FULL"), 0, 0, 0, table.data(), table.size());
+                    } else {
+                        encode(op, code("TABLE_GEN"), ne1, 0, 0, table.data(), table.size());
+                    }
+                    if (ne1) {
+                        emit(op, o.c_wg, o.v_wg);
+                        n_out++;
+                    }
+                    k += o.tbit;
+                    continue;
+                }
+                case DQ_OP_NEG:
+                    encode(op, cond_variant("NEG_ALL", "NEG_B", "NEG_BB", "NEG_GEN", o.c_reg, o.v_reg),
+                           jmask(o.c_reg, o.v_reg, -1), o.c_thr, o.v_thr, nullptr, 0);
+                    break;
+                case DQ_OP_PHASE:
+                    encode(op, cond_variant("PHASE_ALL", "PHASE_B", "PHASE_BB", "PHASE_GEN", o.c_reg, o.v_reg),
+                           jmask(o.c_reg, o.v_reg, -1), o.c_thr, o.v_thr, m, 2 * ts);
+                    break;
+                case DQ_OP_MAT:
+                    encode(op, code("MAT", (int)o.tbit), jmask(o.c_reg, o.v_reg, (int)o.tbit), o.c_thr, o.v_thr, m, 8 * ts);
+                    break;
+                case DQ_OP_SWAP:
+                    encode(op, code("SWAP", (int)o.tbit), jmask(o.c_reg, o.v_reg, (int)o.tbit), o.c_thr, o.v_thr, nullptr, 0);
+                    break;
+                case DQ_OP_RMAT:
+                    encode(op, code("RMAT", (int)o.tbit), 0, 0, 0, m, 4 * ts);
+                    break;
+                default:
+                    continue;
+            }
+            emit(op, o.c_wg, o.v_wg);
+            n_out++;
+        }
+        op.clear();
+        encode(op, code("END"), 0, 0, 0, nullptr, 0);
+        out.insert(out.end(), op.begin(), op.end());
+    }
+    // Walk a built program: every code in range, sizes chaining to END at the
+    // end of the buffer (the interpreter jumps through a table on the code).
+    bool check(size_t begin) const {
+        size_t at = begin;
+        while (at + 16 <= out.size()) {
+            uint32_t c;
+            memcpy(&c, out.data() + at, 4);
+            if (c >= (uint32_t)ncodes) return false;
+            if (!strcmp(tab[c].name, "END")) return at + size(c) == out.size();
+            at += size(c);
+        }
+        return false;
+    }
+};
+}  // namespace
 
 extern "C" int dq_init_state(void *state, int n, int prec, int kind, uint64_t index, double value, void *stream) {
     if (!state || n < 1 || n > 40 || (prec != DQ_F32 && prec != DQ_F64)) return fail(DQ_ERR_ARG, "dq_init_state: bad arguments");
@@ -918,10 +1352,13 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             if (prog_r[p] >= 0) return fail(DQ_ERR_ARG, "phase %u shared by blocks of different register bits", p);
             prog_r[p] = (int)b.reg_bits;
             pprog[2 * p] = (uint32_t)prog.size();
-            build_prog(prec, (int)b.reg_bits, ops + phases[p].op_begin, phases[p].op_count, prog, pprog[2 * p + 1]);
+            ProgBuilder pb(prec, (int)b.reg_bits, prog);
+            const size_t start = prog.size();
+            pb.build(ops + phases[p].op_begin, phases[p].op_count, pprog[2 * p + 1]);
+            if (pb.bad || !pb.check(start)) return fail(DQ_ERR_ARG, "phase %u: op not expressible as a device op", p);
         }
     }
-    if (prog.size() + 256 > prog_bound(n_ops)) return fail(DQ_ERR_ARG, "dq_apply_blocks: device program overflow");
+    if (prog.size() + 256 > prog_bound(n_phases, n_ops)) return fail(DQ_ERR_ARG, "dq_apply_blocks: device program overflow");
 
     // pack and upload
     std::vector<unsigned char> buf(need, 0);
@@ -1023,6 +1460,154 @@ extern "C" int dq_prob_zero_all(const void *state, int n, int prec, double *out,
     if (rc) return rc;
     HIPC(hipMemcpyAsync(out, d_out, (n + 1) * sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream));
     HIPC(hipStreamSynchronize((hipStream_t)stream));
+    return DQ_OK;
+}
+
+// ---- read-out of many states ------------------------------------------------
+namespace {
+struct RoPlan {
+    int gb = 0;
+    uint64_t wgs = 0;  // stage-1 workgroups per state
+    RoLevels lv;
+    size_t off_ptr = 0, off_u = 0, off_part = 0, off_lvl[RO_MAX_LEVELS] = {0, 0, 0, 0}, total = 0;
+};
+
+RoPlan ro_plan(int n, uint32_t count, uint32_t shots) {
+    RoPlan p;
+    memset(&p.lv, 0, sizeof p.lv);
+    p.gb = n < RO_GB ? n : RO_GB;
+    const uint64_t n1 = 1ull << (n - p.gb);
+    p.wgs = (n1 + 4 * RO_GPW - 1) / (4 * RO_GPW);
+    size_t off = 0;
+    p.off_ptr = off;
+    off += align_up((size_t)count * sizeof(void *), 256);
+    p.off_u = off;
+    off += align_up((size_t)count * shots * sizeof(double), 256);
+    p.off_part = off;
+    off += align_up((size_t)count * p.wgs * PROB_SLOTS * sizeof(double), 256);
+    uint64_t c = n1;
+    int L = 0;
+    for (;;) {
+        p.lv.count[L] = c;
+        p.lv.stride[L] = c;
+        p.off_lvl[L] = off;
+        off += align_up((size_t)count * c * sizeof(double), 256);
+        L++;
+        if (c <= RO_FAN) break;
+        c = (c + RO_FAN - 1) >> RO_FAN_BITS;
+    }
+    p.lv.nlev = L;
+    p.total = off;
+    return p;
+}
+}  // namespace
+
+extern "C" size_t dq_readout_ws_bytes(int n, uint32_t count, uint32_t shots) {
+    if (n < 1 || n > 40 || count == 0) return 0;
+    return ro_plan(n, count, shots).total;
+}
+
+extern "C" int dq_readout_many(int prec, int n, const void *const *states, uint32_t count, const double *u,
+                               uint32_t shots, double *d_p0, uint64_t *d_shots, void *d_ws, size_t ws_bytes,
+                               void *stream) {
+    if ((prec != DQ_F32 && prec != DQ_F64) || n < 1 || n > 40 || !states || count == 0 || count > 65535)
+        return fail(DQ_ERR_ARG, "dq_readout_many: bad arguments");
+    if (shots > (1u << 24)) return fail(DQ_ERR_ARG, "dq_readout_many: at most 2^24 samples per state");
+    if (shots && (!u || !d_shots)) return fail(DQ_ERR_ARG, "dq_readout_many: samples need u and d_shots");
+    for (uint32_t i = 0; i < count; i++)
+        if (!states[i]) return fail(DQ_ERR_ARG, "dq_readout_many: state %u is null", i);
+    for (uint64_t i = 0; i < (uint64_t)count * shots; i++)
+        if (!(u[i] >= 0.0 && u[i] < 1.0)) return fail(DQ_ERR_ARG, "dq_readout_many: u[%llu] not in [0, 1)", (unsigned long long)i);
+    const RoPlan p = ro_plan(n, count, shots);
+    if (!d_ws || ws_bytes < p.total) return fail(DQ_ERR_ARG, "dq_readout_many: workspace %zu < %zu bytes", ws_bytes, p.total);
+    if (p.wgs > 0x7fffffffull) return fail(DQ_ERR_UNSUPPORTED, "dq_readout_many: grid too large");
+    // state pointers and uniforms travel in one staged copy
+    std::vector<unsigned char> buf(p.off_part - p.off_ptr, 0);
+    memcpy(buf.data(), states, (size_t)count * sizeof(void *));
+    if (shots) memcpy(buf.data() + (p.off_u - p.off_ptr), u, (size_t)count * shots * sizeof(double));
+    unsigned char *d = (unsigned char *)d_ws;
+    int rc = stage_upload(d + p.off_ptr, buf, stream);
+    if (rc) return rc;
+    const void *const *d_states = (const void *const *)(d + p.off_ptr);
+    double *part = (double *)(d + p.off_part);
+    RoLevels lv = p.lv;
+    for (int L = 0; L < lv.nlev; L++) lv.base[L] = (double *)(d + p.off_lvl[L]);
+    const uint64_t part_stride = p.wgs * PROB_SLOTS;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == DQ_F64)
+        hipLaunchKernelGGL(readout_stage1<double>, dim3((unsigned)p.wgs, count), dim3(256), 0, s, d_states, n, part,
+                           part_stride, lv.base[0], lv.stride[0]);
+    else
+        hipLaunchKernelGGL(readout_stage1<float>, dim3((unsigned)p.wgs, count), dim3(256), 0, s, d_states, n, part,
+                           part_stride, lv.base[0], lv.stride[0]);
+    HIPC(hipGetLastError());
+    if (d_p0) {
+        hipLaunchKernelGGL(readout_stage2, dim3(n + 1, count), dim3(256), 0, s, part, part_stride, (uint32_t)p.wgs, n, d_p0);
+        HIPC(hipGetLastError());
+    }
+    if (!shots) return DQ_OK;
+    for (int L = 1; L < lv.nlev; L++) {
+        hipLaunchKernelGGL(readout_level, dim3((unsigned)((lv.count[L] + 3) / 4), count), dim3(256), 0, s,
+                           lv.base[L - 1], lv.count[L - 1], lv.stride[L - 1], lv.base[L], lv.count[L], lv.stride[L]);
+        HIPC(hipGetLastError());
+    }
+    const double *d_u = (const double *)(d + p.off_u);
+    if (prec == DQ_F64)
+        hipLaunchKernelGGL(readout_sample<double>, dim3((shots + 3) / 4, count), dim3(256), 0, s, d_states, n, lv, d_u,
+                           shots, d_shots);
+    else
+        hipLaunchKernelGGL(readout_sample<float>, dim3((shots + 3) / 4, count), dim3(256), 0, s, d_states, n, lv, d_u,
+                           shots, d_shots);
+    HIPC(hipGetLastError());
+    return DQ_OK;
+}
+
+// ---- comparison of two states -------------------------------------------------
+extern "C" size_t dq_compare_ws_bytes(int n) {
+    if (n < 1 || n > 40) return 0;
+    const int cb = n < PROB_WG_BITS ? n : PROB_WG_BITS;
+    return align_up(((size_t)1 << (n - cb)) * CMP_SLOTS * sizeof(double), 256);
+}
+
+template <typename TA, typename TB>
+static void launch_compare(const void *a, const void *b, int n, unsigned groups, double *part, hipStream_t s) {
+    hipLaunchKernelGGL((compare_stage1<TA, TB>), dim3(groups), dim3(256), 0, s, (const typename V2<TA>::type *)a,
+                       (const typename V2<TB>::type *)b, n, part);
+}
+
+extern "C" int dq_compare_states(const void *a, int prec_a, const void *b, int prec_b, int n, double *d_out,
+                                 void *d_ws, size_t ws_bytes, void *stream) {
+    const bool pa = prec_a == DQ_F32 || prec_a == DQ_F64, pb = prec_b == DQ_F32 || prec_b == DQ_F64;
+    if (!a || !b || !d_out || !pa || !pb || n < 1 || n > 40) return fail(DQ_ERR_ARG, "dq_compare_states: bad arguments");
+    if (!d_ws || ws_bytes < dq_compare_ws_bytes(n)) return fail(DQ_ERR_ARG, "dq_compare_states: workspace too small");
+    const int cb = n < PROB_WG_BITS ? n : PROB_WG_BITS;
+    const uint64_t groups = 1ull << (n - cb);
+    double *part = (double *)d_ws;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec_a == DQ_F64 && prec_b == DQ_F64) launch_compare<double, double>(a, b, n, (unsigned)groups, part, s);
+    else if (prec_a == DQ_F64) launch_compare<double, float>(a, b, n, (unsigned)groups, part, s);
+    else if (prec_b == DQ_F64) launch_compare<float, double>(a, b, n, (unsigned)groups, part, s);
+    else launch_compare<float, float>(a, b, n, (unsigned)groups, part, s);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(compare_stage2, dim3(1), dim3(256), 0, s, part, (uint32_t)groups, d_out);
+    HIPC(hipGetLastError());
+    return DQ_OK;
+}
+
+extern "C" int dq_event_query(void *ev) {
+    if (!ev) return fail(DQ_ERR_ARG, "dq_event_query: null event");
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: keep it out of the next launch check
+        return 1;
+    }
+    return fail(DQ_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(e));
+}
+
+extern "C" int dq_event_sync(void *ev) {
+    if (!ev) return fail(DQ_ERR_ARG, "dq_event_sync: null event");
+    HIPC(hipEventSynchronize((hipEvent_t)ev));
     return DQ_OK;
 }
 

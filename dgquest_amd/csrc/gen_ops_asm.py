@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Generates ops_asm.inc: the op interpreter of the sweep kernel in gfx950
-assembly, one inline-asm body per (amplitude type, register bits R).
+assembly (one inline-asm body per amplitude type and register bits R) and
+the op-code table the engine's host code uses to build device programs.
 
 Why assembly: the register tile of a thread (2^R complex amplitudes) must be
 updated in place by a loop over a runtime op list.  Compiled from C++, the
@@ -10,69 +11,110 @@ which costs more VALU issue than the arithmetic itself.  Inside one asm
 statement every tile value is a tied "+v" operand, so it lives in one
 register for the whole op list.
 
-Device program format (built by the engine's host code from dq_op arrays):
-each op is a 32-byte header followed by coefficients of the amplitude type,
-padded to a multiple of 32 bytes:
-    u32 disp    kind * 8 + register bit
-    u32 jmask   register indices j that satisfy the op's register condition
-                (pair ops: the lower index of each pair; DIAGR: entries != 1)
-    u32 thr     lane condition: c_thr | v_thr << 16 (c_thr = 0: every lane)
-    u32 size    bytes of this op, header included
-    u64 c_wg    workgroup condition: (tile base & c_wg) == v_wg
-    u64 v_wg
-    T   coef[]  SHEAR u, l | PHASE re, im | RMAT m0..m3 | MAT u00..u11 (re, im)
-                | DIAGR 2^R complex factors | NEG, SWAP: none
-Kinds (device numbering): 1 SHEAR, 2 DIAGR, 3 NEG, 4 PHASE, 5 MAT, 6 SWAP, 7 RMAT.
+Why few scalar instructions: a CU issues one scalar instruction per cycle for
+all its waves, as many as its four SIMDs issue 64-lane vector instructions,
+so per-op bookkeeping (decoding, branches, pointer updates) competes with the
+arithmetic.  Ops are dispatched through a jump table, have fixed sizes per op
+code, end the list with an END op, carry workgroup conditions in a separate
+WG op only when they have one, and come in variants whose register condition
+is resolved at generation time (no per-register branches).
 
-Scalar registers s36..s68 are reserved by the asm (clobbered; the compiler
+Device program (built by the engine's host code from dq_op arrays): each op is
+a 16-byte header followed by its coefficients in the amplitude type; the
+size of each op code is fixed (op_size below).
+    u32 code     index into the jump table
+    u32 mask     SHEARS: register bits sheared; *_GEN / MAT / SWAP: register
+                 indices (pair ops: lower indices) that satisfy the condition
+    u32 c_thr    lane condition mask (0: every lane) ...
+    u32 v_thr    ... and value; WG: bytes of the guarded op
+    coefficients from byte 16:
+      WG        c_wg (u64), v_wg (u64)
+      SHEARS    per register bit b: u_b, l_b at 16 + 2 * b * sizeof(T)
+      TABLE_*   2^R factors (HALF: the 2^(R-1) with the bit set, in order)
+      PHASE_*   re, im          MAT  u00 u01 u10 u11 (re, im)     RMAT m0..m3
+Scalar registers s36..s71 are reserved by the asm (clobbered; the compiler
 keeps the SGPRs above s71 for itself at high occupancy):
-    s[36:37] scratch    s[38:39] saved exec  s[40:41] next op  s[42:43] current op
-    s[44:51] second coefficient buffer (FP64 MAT u10, u11)
-    s[52:67] header + first 32 bytes of coefficients (also DIAGR table chunks)
-    s68      ops left
+    s[36:51] second load buffer / scratch    s[52:67] header + first coefficients
+    s[68:69] next op                          s[70:71] saved exec
 Only scalar loads touch memory; every store is a vector register write.
 
 Usage: python3 gen_ops_asm.py > ops_asm.inc   (the Makefile does this)
 """
+import itertools
 import sys
 
-KIND = {"SHEAR": 1, "DIAGR": 2, "NEG": 3, "PHASE": 4, "MAT": 5, "SWAP": 6, "RMAT": 7}
-
-# scalar register map (see the module docstring)
-SCR, EXEC, NEXT, CUR, BUF2, HDR, CNT = 36, 38, 40, 42, 44, 52, 68
-COEF = HDR + 8          # first coefficient right after the 32-byte header
-CLOBBER = range(36, 69)
+BUF2, HDR, NEXT, EXEC = 36, 52, 68, 70
+COEF = HDR + 4          # coefficients start right after the 16-byte header
+CLOBBER = range(36, 72)
+RMAX = 5
 
 
 def sp(b):
     return f"s[{b}:{b + 1}]"
 
 
+def op_codes(R):
+    """Op codes, the same list for every amplitude type: (name, params)."""
+    codes = [("END", ()), ("WG", ()), ("SHEARS", ()), ("TABLE_FULL", ()), ("TABLE_GEN", ())]
+    codes += [("TABLE_HALF", (r,)) for r in range(R)]
+    for k in ("NEG", "PHASE"):
+        codes += [(f"{k}_ALL", ()), (f"{k}_GEN", ())]
+        codes += [(f"{k}_B", (r, v)) for r in range(R) for v in (0, 1)]
+        codes += [(f"{k}_BB", (r1, r2, v1, v2)) for r1, r2 in itertools.combinations(range(R), 2)
+                  for v1 in (0, 1) for v2 in (0, 1)]
+    codes += [("MAT", (b,)) for b in range(R)]
+    codes += [("SWAP", (b,)) for b in range(R)]
+    codes += [("RMAT", (b,)) for b in range(R)]
+    return codes
+
+
+def op_size(name, R, ts):
+    """Bytes of an op (a multiple of 16)."""
+    base = name.split("_")[0]
+    if name in ("END", "SWAP") or base == "NEG":
+        b = 0
+    elif name == "WG":
+        b = 16
+    elif name == "SHEARS":
+        b = 2 * R * ts
+    elif name in ("TABLE_FULL", "TABLE_GEN"):
+        b = (2 << R) * ts
+    elif name == "TABLE_HALF":
+        b = (1 << R) * ts
+    elif base == "PHASE":
+        b = 2 * ts
+    elif name == "MAT":
+        b = 8 * ts
+    elif name == "RMAT":
+        b = 4 * ts
+    else:
+        raise ValueError(name)
+    return (16 + b + 15) // 16 * 16
+
+
 class Gen:
     def __init__(self, T, R):
         self.T, self.R, self.NR = T, R, 1 << R
         self.d = T == "double"
+        self.ts = 8 if self.d else 4
         self.L = []
+        self.codes = op_codes(R)
 
     def e(self, s):
         self.L.append(s)
 
-    # --- operand and register names ----------------------------------------
     def re(self, j):
         return f"%[re{j}]"
 
     def im(self, j):
         return f"%[im{j}]"
 
-    def sbuf(self, base, i):
+    def sval(self, base, i):
         """Value i (in units of T) of a coefficient buffer starting at s<base>."""
         return sp(base + 2 * i) if self.d else f"s{base + i}"
 
-    def sreg(self, i):
-        return self.sbuf(COEF, i)
-
     @property
-    def sfx(self):
+    def f(self):
         return "f64" if self.d else "f32"
 
     @property
@@ -82,101 +124,188 @@ class Gen:
     def lbl(self, name):
         return f".L{name}_%="
 
-    # --- op bodies ------------------------------------------------------------
+    def tail(self, name):
+        size = op_size(name, self.R, self.ts)
+        self.e(f"s_add_u32 s{NEXT}, s{NEXT}, {size}")
+        self.e(f"s_addc_u32 s{NEXT + 1}, s{NEXT + 1}, 0")
+        self.e(f"s_branch {self.lbl('op')}")
+
+    # --- arithmetic building blocks ------------------------------------------
     def pairs(self, B):
         return [(j0, j0 | (1 << B)) for j0 in range(self.NR) if not (j0 >> B) & 1]
 
-    def shear(self, B):
-        f = self.sfx
-        u, l = self.sreg(0), self.sreg(1)
-        P = self.pairs(B)
-        for j0, j1 in P:
-            self.e(f"v_fmac_{f} {self.re(j0)}, {u}, {self.re(j1)}")
-            self.e(f"v_fmac_{f} {self.im(j0)}, {u}, {self.im(j1)}")
-        for j0, j1 in P:
-            self.e(f"v_fmac_{f} {self.re(j1)}, {l}, {self.re(j0)}")
-            self.e(f"v_fmac_{f} {self.im(j1)}, {l}, {self.im(j0)}")
+    def cmul(self, j, pr, pi):
+        f = self.f
+        self.e(f"v_mul_{f} %[t0], {pi}, {self.im(j)}")
+        self.e(f"v_mul_{f} %[t1], {pi}, {self.re(j)}")
+        self.e(f"v_fma_{f} {self.re(j)}, {pr}, {self.re(j)}, -%[t0]")
+        self.e(f"v_fma_{f} {self.im(j)}, {pr}, {self.im(j)}, %[t1]")
 
-    def cmul(self, js, pr_of, pi_of):
-        """amplitudes js *= (pr + i pi), two at a time (temporaries t0..t3)."""
-        f = self.sfx
+    def cmul_list(self, js, pr_of, pi_of):
+        """two amplitudes at a time (temporaries t0..t3) so products overlap."""
+        f = self.f
         for k in range(0, len(js), 2):
             grp = js[k:k + 2]
             for n, j in enumerate(grp):
-                pi = pi_of(j)
-                self.e(f"v_mul_{f} %[t{2 * n}], {pi}, {self.im(j)}")
-                self.e(f"v_mul_{f} %[t{2 * n + 1}], {pi}, {self.re(j)}")
+                self.e(f"v_mul_{f} %[t{2 * n}], {pi_of(j)}, {self.im(j)}")
+                self.e(f"v_mul_{f} %[t{2 * n + 1}], {pi_of(j)}, {self.re(j)}")
             for n, j in enumerate(grp):
-                pr = pr_of(j)
-                self.e(f"v_fma_{f} {self.re(j)}, {pr}, {self.re(j)}, -%[t{2 * n}]")
-                self.e(f"v_fma_{f} {self.im(j)}, {pr}, {self.im(j)}, %[t{2 * n + 1}]")
+                self.e(f"v_fma_{f} {self.re(j)}, {pr_of(j)}, {self.re(j)}, -%[t{2 * n}]")
+                self.e(f"v_fma_{f} {self.im(j)}, {pr_of(j)}, {self.im(j)}, %[t{2 * n + 1}]")
 
-    def diagr(self):
-        # table at current op + 32, read in 64-byte chunks into the header
-        # buffer (the header is no longer needed once the op is dispatched)
-        # (jmask, moved to s[SCR] before the header buffer is reused, skips the
-        # registers whose factor is exactly 1)
-        per = 4 if self.d else 8          # complex entries per 64-byte chunk
-        nch = (self.NR + per - 1) // per
-        f = self.sfx
-        self.e(f"s_mov_b32 s{SCR}, s{HDR + 1}")
-        for c in range(nch):
-            self.e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], {sp(CUR)}, {hex(0x20 + 64 * c)}")
-            self.e("s_waitcnt lgkmcnt(0)")
-            for j in range(c * per, min(self.NR, (c + 1) * per)):
-                pr, pi = self.sbuf(HDR, 2 * (j % per)), self.sbuf(HDR, 2 * (j % per) + 1)
-                tag = f"dg{j}"
-                self.e(f"s_bitcmp1_b32 s{SCR}, {j}")
-                self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
-                self.e(f"v_mul_{f} %[t0], {pi}, {self.im(j)}")
-                self.e(f"v_mul_{f} %[t1], {pi}, {self.re(j)}")
-                self.e(f"v_fma_{f} {self.re(j)}, {pr}, {self.re(j)}, -%[t0]")
-                self.e(f"v_fma_{f} {self.im(j)}, {pr}, {self.im(j)}, %[t1]")
-                self.e(f"{self.lbl(tag)}:")
+    def lanes_on(self, tag):
+        """exec &= lanes with (tid & c_thr) == v_thr; exec saved in s[70:71]."""
+        self.e(f"s_mov_b64 {sp(EXEC)}, exec")
+        self.e(f"s_cmp_eq_u32 s{HDR + 2}, 0")
+        self.e(f"s_cbranch_scc1 {self.lbl(tag)}")
+        self.e(f"v_and_b32 %[vt], s{HDR + 2}, %[tid]")
+        self.e(f"v_cmp_eq_u32 vcc, s{HDR + 3}, %[vt]")
+        self.e("s_and_b64 exec, exec, vcc")
+        self.e(f"{self.lbl(tag)}:")
+
+    def lanes_off(self):
+        self.e(f"s_mov_b64 exec, {sp(EXEC)}")
 
     def jskip(self, j, tag):
         self.e(f"s_bitcmp1_b32 s{HDR + 1}, {j}")
         self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
 
-    def lanes_on(self, tag):
-        """exec &= lanes satisfying (tid & c_thr) == v_thr (exec is saved before
-        dispatch and restored after every op)."""
-        self.e(f"s_and_b32 s{SCR}, s{HDR + 2}, 0xffff")
-        self.e(f"s_cmp_eq_u32 s{SCR}, 0")
-        self.e(f"s_cbranch_scc1 {self.lbl(tag)}")
-        self.e(f"s_lshr_b32 s{SCR + 1}, s{HDR + 2}, 16")
-        self.e(f"v_and_b32 %[vt], s{SCR}, %[tid]")
-        self.e(f"v_cmp_eq_u32 vcc, s{SCR + 1}, %[vt]")
-        self.e("s_and_b64 exec, exec, vcc")
-        self.e(f"{self.lbl(tag)}:")
+    # --- table factors (starting at op + 16) in 64-byte chunks alternating
+    # between s[52:67] and s[36:51]; scalar loads return out of order, so
+    # every wait is lgkmcnt(0) -----------------------------------------------------
+    def table_apply(self, entries):
+        """entries: (register index j, position of its factor in the stored
+        table).  Loads only the chunks that hold a needed factor."""
+        per = 4 if self.d else 8
+        chunks = sorted({p // per for _, p in entries})
+        bufs = [HDR, BUF2]
+        where = {}
+        for i, c in enumerate(chunks[:2]):
+            self.e(f"s_load_dwordx16 s[{bufs[i]}:{bufs[i] + 15}], {sp(NEXT)}, {hex(16 + 64 * c)}")
+            where[c] = bufs[i]
+        self.e("s_waitcnt lgkmcnt(0)")
+        for i, c in enumerate(chunks):
+            buf = where[c]
+            pos = {j: p for j, p in entries if p // per == c}
+            js = sorted(pos)
+            self.cmul_list(js, lambda j, buf=buf, pos=pos: self.sval(buf, 2 * (pos[j] % per)),
+                           lambda j, buf=buf, pos=pos: self.sval(buf, 2 * (pos[j] % per) + 1))
+            if i + 2 < len(chunks):  # refill this buffer with the chunk after next
+                c2 = chunks[i + 2]
+                self.e(f"s_load_dwordx16 s[{buf}:{buf + 15}], {sp(NEXT)}, {hex(16 + 64 * c2)}")
+                where[c2] = buf
+            if i % 2 == 1 and i + 1 < len(chunks):
+                self.e("s_waitcnt lgkmcnt(0)")
 
-    def neg(self):
-        f = self.sfx
-        for j in range(self.NR):
-            tag = f"ng{j}"
-            self.jskip(j, tag)
+    # --- handlers ---------------------------------------------------------------
+    def h_wg(self):
+        # skip the guarded op (its size in the header) unless (base & c_wg) == v_wg
+        self.e(f"s_and_b64 s[36:37], %[base], {sp(COEF)}")
+        self.e(f"s_cmp_eq_u64 s[36:37], {sp(COEF + 2)}")
+        self.e(f"s_cselect_b32 s38, 0, s{HDR + 3}")
+        self.e(f"s_add_u32 s{NEXT}, s{NEXT}, s38")
+        self.e(f"s_addc_u32 s{NEXT + 1}, s{NEXT + 1}, 0")
+        self.tail("WG")
+
+    def h_shears(self):
+        f = self.f
+        if self.d and self.R >= 4:  # slots 3 (and 4) beyond the first load
+            self.e(f"s_load_dwordx8 s[{BUF2}:{BUF2 + 7}], {sp(NEXT)}, 0x40")
+            self.e("s_waitcnt lgkmcnt(0)")
+        for B in range(self.R):
+            if self.d:
+                if B < 3:
+                    u, l = sp(COEF + 4 * B), sp(COEF + 4 * B + 2)
+                else:
+                    u, l = sp(BUF2 + 4 * (B - 3)), sp(BUF2 + 4 * (B - 3) + 2)
+            else:
+                u, l = f"s{COEF + 2 * B}", f"s{COEF + 2 * B + 1}"
+            tag = f"sh{B}"
+            self.e(f"s_bitcmp1_b32 s{HDR + 1}, {B}")
+            self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
+            P = self.pairs(B)
+            for j0, j1 in P:
+                self.e(f"v_fmac_{f} {self.re(j0)}, {u}, {self.re(j1)}")
+                self.e(f"v_fmac_{f} {self.im(j0)}, {u}, {self.im(j1)}")
+            for j0, j1 in P:
+                self.e(f"v_fmac_{f} {self.re(j1)}, {l}, {self.re(j0)}")
+                self.e(f"v_fmac_{f} {self.im(j1)}, {l}, {self.im(j0)}")
+            self.e(f"{self.lbl(tag)}:")
+        self.tail("SHEARS")
+
+    def h_table(self, name, params):
+        if name == "TABLE_FULL":
+            self.table_apply([(j, j) for j in range(self.NR)])
+        elif name == "TABLE_HALF":
+            (r,) = params
+            js = [j for j in range(self.NR) if (j >> r) & 1]
+            self.table_apply([(j, i) for i, j in enumerate(js)])
+        else:  # TABLE_GEN: per-register skip on the mask (kept in s70 while loading)
+            self.e(f"s_mov_b32 s{EXEC}, s{HDR + 1}")
+            per = 4 if self.d else 8
+            for c in range((self.NR + per - 1) // per):
+                self.e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], {sp(NEXT)}, {hex(16 + 64 * c)}")
+                self.e("s_waitcnt lgkmcnt(0)")
+                for j in range(c * per, min(self.NR, (c + 1) * per)):
+                    tag = f"tg{j}"
+                    self.e(f"s_bitcmp1_b32 s{EXEC}, {j}")
+                    self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
+                    self.cmul(j, self.sval(HDR, 2 * (j % per)), self.sval(HDR, 2 * (j % per) + 1))
+                    self.e(f"{self.lbl(tag)}:")
+        self.tail(name)
+
+    def sel(self, name, params):
+        """register indices an ALL / B / BB variant acts on (None: GEN, the mask)."""
+        kind = name.split("_")[1]
+        if kind == "ALL":
+            return list(range(self.NR))
+        if kind == "B":
+            r, v = params
+            return [j for j in range(self.NR) if ((j >> r) & 1) == v]
+        if kind == "BB":
+            r1, r2, v1, v2 = params
+            return [j for j in range(self.NR) if ((j >> r1) & 1) == v1 and ((j >> r2) & 1) == v2]
+        return None
+
+    def h_neg(self, name, params, tag):
+        f = self.f
+        self.lanes_on(tag + "l")
+        js = self.sel(name, params)
+        for j in (js if js is not None else range(self.NR)):
+            if js is None:
+                self.jskip(j, f"{tag}j{j}")
             self.e(f"v_mul_{f} {self.re(j)}, -1.0, {self.re(j)}")
             self.e(f"v_mul_{f} {self.im(j)}, -1.0, {self.im(j)}")
-            self.e(f"{self.lbl(tag)}:")
+            if js is None:
+                self.e(f"{self.lbl(f'{tag}j{j}')}:")
+        self.lanes_off()
+        self.tail(name)
 
-    def phase(self):
-        f = self.sfx
-        pr, pi = self.sreg(0), self.sreg(1)
-        for j in range(self.NR):
-            tag = f"ph{j}"
-            self.jskip(j, tag)
-            self.e(f"v_mul_{f} %[t0], {pi}, {self.im(j)}")
-            self.e(f"v_mul_{f} %[t1], {pi}, {self.re(j)}")
-            self.e(f"v_fma_{f} {self.re(j)}, {pr}, {self.re(j)}, -%[t0]")
-            self.e(f"v_fma_{f} {self.im(j)}, {pr}, {self.im(j)}, %[t1]")
-            self.e(f"{self.lbl(tag)}:")
-
-    def mat(self, B):
-        f = self.sfx
-        if self.d:
-            u = [self.sreg(i) for i in range(4)] + [self.sbuf(BUF2, i) for i in range(4)]
+    def h_phase(self, name, params, tag):
+        self.lanes_on(tag + "l")
+        pr, pi = self.sval(COEF, 0), self.sval(COEF, 1)
+        js = self.sel(name, params)
+        if js is not None:
+            self.cmul_list(js, lambda j: pr, lambda j: pi)
         else:
-            u = [self.sreg(i) for i in range(8)]
+            for j in range(self.NR):
+                self.jskip(j, f"{tag}j{j}")
+                self.cmul(j, pr, pi)
+                self.e(f"{self.lbl(f'{tag}j{j}')}:")
+        self.lanes_off()
+        self.tail(name)
+
+    def h_mat(self, params):
+        (B,) = params
+        f = self.f
+        if self.d:  # u11 (re, im) lies beyond the first load
+            self.e(f"s_load_dwordx4 s[{BUF2}:{BUF2 + 3}], {sp(NEXT)}, 0x40")
+            u = [self.sval(COEF, i) for i in range(6)] + [sp(BUF2), sp(BUF2 + 2)]
+        else:
+            u = [self.sval(COEF, i) for i in range(8)]
+        self.lanes_on(f"mt{B}l")
+        if self.d:
+            self.e("s_waitcnt lgkmcnt(0)")
         u0r, u0i, u1r, u1i, u2r, u2i, u3r, u3i = u
         for j0, j1 in self.pairs(B):
             tag = f"mt{B}_{j0}"
@@ -202,8 +331,12 @@ class Gen:
             self.e(f"{self.mov} {xi}, %[t1]")
             self.e(f"{self.mov} {yr}, %[t2]")
             self.e(f"{self.lbl(tag)}:")
+        self.lanes_off()
+        self.tail("MAT")
 
-    def swap(self, B):
+    def h_swap(self, params):
+        (B,) = params
+        self.lanes_on(f"sw{B}l")
         for j0, j1 in self.pairs(B):
             tag = f"sw{B}_{j0}"
             self.jskip(j0, tag)
@@ -212,10 +345,13 @@ class Gen:
                 self.e(f"{self.mov} {a}, {b}")
                 self.e(f"{self.mov} {b}, %[t0]")
             self.e(f"{self.lbl(tag)}:")
+        self.lanes_off()
+        self.tail("SWAP")
 
-    def rmat(self, B):
-        f = self.sfx
-        m0, m1, m2, m3 = (self.sreg(i) for i in range(4))
+    def h_rmat(self, params):
+        (B,) = params
+        f = self.f
+        m0, m1, m2, m3 = (self.sval(COEF, i) for i in range(4))
         for j0, j1 in self.pairs(B):
             xr, xi, yr, yi = self.re(j0), self.im(j0), self.re(j1), self.im(j1)
             self.e(f"v_mul_{f} %[t0], {m0}, {xr}")
@@ -228,78 +364,47 @@ class Gen:
             self.e(f"v_fma_{f} {yi}, {m3}, {yi}, %[t3]")
             self.e(f"{self.mov} {xr}, %[t0]")
             self.e(f"{self.mov} {xi}, %[t1]")
-
-    def per_bit(self, kind, body):
-        """dispatch on the register bit (header dword 0 = kind * 8 + bit), then body(B)."""
-        for B in range(self.R):
-            self.e(f"s_cmp_eq_u32 s{HDR}, {KIND[kind] * 8 + B}")
-            self.e(f"s_cbranch_scc1 {self.lbl(f'{kind}{B}')}")
-        self.e(f"s_branch {self.lbl('done')}")
-        for B in range(self.R):
-            self.e(f"{self.lbl(f'{kind}{B}')}:")
-            body(B)
-            self.e(f"s_branch {self.lbl('done')}")
+        self.tail("RMAT")
 
     # --- the whole interpreter -----------------------------------------------
     def build(self):
         e = self.e
         e(f"s_mov_b64 {sp(NEXT)}, %[prog]")
-        e(f"s_mov_b32 s{CNT}, %[nops]")
-        e(f"s_cmp_eq_u32 s{CNT}, 0")
-        e(f"s_cbranch_scc1 {self.lbl('end')}")
         e(f"{self.lbl('op')}:")
         e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], {sp(NEXT)}, 0x0")
         e("s_waitcnt lgkmcnt(0)")
-        e(f"s_mov_b64 {sp(CUR)}, {sp(NEXT)}")
-        e(f"s_add_u32 s{NEXT}, s{NEXT}, s{HDR + 3}")
-        e(f"s_addc_u32 s{NEXT + 1}, s{NEXT + 1}, 0")
-        e(f"s_sub_u32 s{CNT}, s{CNT}, 1")
-        e(f"s_and_b64 {sp(SCR)}, %[base], {sp(HDR + 4)}")
-        e(f"s_cmp_eq_u64 {sp(SCR)}, {sp(HDR + 6)}")
-        e(f"s_cbranch_scc0 {self.lbl('next')}")
-        e(f"s_mov_b64 {sp(EXEC)}, exec")
-        for kind in ("SHEAR", "DIAGR", "NEG", "PHASE", "MAT", "SWAP", "RMAT"):
-            e(f"s_cmp_lt_u32 s{HDR}, {8 * (KIND[kind] + 1)}")
-            e(f"s_cbranch_scc1 {self.lbl('k' + kind)}")
-        e(f"s_branch {self.lbl('next')}")
-
-        e(f"{self.lbl('kSHEAR')}:")
-        self.per_bit("SHEAR", self.shear)
-
-        e(f"{self.lbl('kDIAGR')}:")
-        self.diagr()
-        e(f"s_branch {self.lbl('done')}")
-
-        e(f"{self.lbl('kNEG')}:")
-        self.lanes_on("nglanes")
-        self.neg()
-        e(f"s_branch {self.lbl('done')}")
-
-        e(f"{self.lbl('kPHASE')}:")
-        self.lanes_on("phlanes")
-        self.phase()
-        e(f"s_branch {self.lbl('done')}")
-
-        e(f"{self.lbl('kMAT')}:")
-        if self.d:  # u10, u11 from the second 32 bytes of coefficients
-            e(f"s_load_dwordx8 s[{BUF2}:{BUF2 + 7}], {sp(CUR)}, 0x40")
-        self.lanes_on("mtlanes")
-        if self.d:
-            e("s_waitcnt lgkmcnt(0)")
-        self.per_bit("MAT", self.mat)
-
-        e(f"{self.lbl('kSWAP')}:")
-        self.lanes_on("swlanes")
-        self.per_bit("SWAP", self.swap)
-
-        e(f"{self.lbl('kRMAT')}:")
-        self.per_bit("RMAT", self.rmat)
-
-        e(f"{self.lbl('done')}:")
-        e(f"s_mov_b64 exec, {sp(EXEC)}")
-        e(f"{self.lbl('next')}:")
-        e(f"s_cmp_lg_u32 s{CNT}, 0")
-        e(f"s_cbranch_scc1 {self.lbl('op')}")
+        # jump table: target = .Ltab + 4 * code (every entry is one s_branch)
+        e("s_getpc_b64 s[36:37]")
+        e(f"{self.lbl('pc')}:")
+        e(f"s_lshl2_add_u32 s36, s{HDR}, s36")
+        e("s_addc_u32 s37, s37, 0")
+        e(f"s_add_u32 s36, s36, ({self.lbl('tab')} - {self.lbl('pc')})")
+        e("s_addc_u32 s37, s37, 0")
+        e("s_setpc_b64 s[36:37]")
+        e(f"{self.lbl('tab')}:")
+        for i in range(len(self.codes)):
+            e(f"s_branch {self.lbl(f'h{i}')}")
+        for i, (name, params) in enumerate(self.codes):
+            e(f"{self.lbl(f'h{i}')}:")
+            tag = f"c{i}"
+            if name == "END":
+                e(f"s_branch {self.lbl('end')}")
+            elif name == "WG":
+                self.h_wg()
+            elif name == "SHEARS":
+                self.h_shears()
+            elif name.startswith("TABLE"):
+                self.h_table(name, params)
+            elif name.startswith("NEG"):
+                self.h_neg(name, params, tag)
+            elif name.startswith("PHASE"):
+                self.h_phase(name, params, tag)
+            elif name == "MAT":
+                self.h_mat(params)
+            elif name == "SWAP":
+                self.h_swap(params)
+            elif name == "RMAT":
+                self.h_rmat(params)
         e(f"{self.lbl('end')}:")
         return self.L
 
@@ -308,7 +413,7 @@ class Gen:
         out = []
         out.append(f"template <> struct AsmOps<{self.T}, {self.R}> {{")
         out.append(f"    __device__ __forceinline__ static void run(Regs<{self.T}, {self.R}> &a, const void *prog,")
-        out.append("                                               uint32_t nops, uint64_t base, uint32_t tid) {")
+        out.append("                                               uint64_t base, uint32_t tid) {")
         out.append(f"        {self.T} t0, t1, t2, t3;")
         out.append("        uint32_t vt;")
         out.append("        asm volatile(")
@@ -318,7 +423,7 @@ class Gen:
         outs = [f'[re{j}] "+v"(a.re[{j}])' for j in range(self.NR)] + \
                [f'[im{j}] "+v"(a.im[{j}])' for j in range(self.NR)] + \
                ['[t0] "=&v"(t0)', '[t1] "=&v"(t1)', '[t2] "=&v"(t2)', '[t3] "=&v"(t3)', '[vt] "=&v"(vt)']
-        ins = ['[prog] "s"(prog)', '[nops] "s"(nops)', '[base] "s"(base)', '[tid] "v"(tid)']
+        ins = ['[prog] "s"(prog)', '[base] "s"(base)', '[tid] "v"(tid)']
         out.append("            : " + ",\n              ".join(outs))
         out.append("            : " + ", ".join(ins))
         clob = ", ".join(f'"s{i}"' for i in CLOBBER)
@@ -328,12 +433,40 @@ class Gen:
         return "\n".join(out)
 
 
+def code_table():
+    """C++ view of the op codes for the host-side program builder."""
+    out = ["// op codes of the device program per register bits R (index = code)",
+           "struct DevOpCode {",
+           "    const char *name;",
+           "    int p[4];",
+           "    int size64, size32;  // bytes for double / float amplitudes",
+           "};"]
+    for R in range(1, RMAX + 1):
+        out.append(f"static const DevOpCode kDevOps{R}[] = {{")
+        for name, params in op_codes(R):
+            p = list(params) + [-1] * (4 - len(params))
+            out.append(f'    {{"{name}", {{{p[0]}, {p[1]}, {p[2]}, {p[3]}}}, {op_size(name, R, 8)}, '
+                       f'{op_size(name, R, 4)}}},')
+        out.append("};")
+    out.append("static inline const DevOpCode *dev_ops(int R, int &count) {")
+    out.append("    switch (R) {")
+    for R in range(1, RMAX + 1):
+        out.append(f"        case {R}: count = (int)(sizeof(kDevOps{R}) / sizeof(DevOpCode)); return kDevOps{R};")
+    out.append("        default: count = 0; return nullptr;")
+    out.append("    }")
+    out.append("}")
+    return "\n".join(out)
+
+
 def main():
     print("// Generated by gen_ops_asm.py -- the sweep kernel's op interpreter in gfx950")
-    print("// assembly, one body per (amplitude type, register bits).  Do not edit.")
+    print("// assembly, one body per (amplitude type, register bits), and the op-code")
+    print("// table of its device programs.  Do not edit.")
+    print(code_table())
+    print()
     print("template <typename T, int R> struct AsmOps;")
     for T in ("double", "float"):
-        for R in range(1, 6):
+        for R in range(1, RMAX + 1):
             print(Gen(T, R).emit())
             print()
 

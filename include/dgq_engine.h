@@ -204,6 +204,35 @@ size_t dq_prob_ws_bytes(int n_qubits);
 int dq_prob_zero_all_async(const void *state, int n_qubits, int prec, double *d_out,
                            void *d_ws, size_t ws_bytes, void *stream);
 
+/* Read-out of `count` states of n qubits each (device pointers in the host
+ * array states[]): for every state, P(qubit q = 0) for q < n and the total
+ * probability (n + 1 doubles at d_p0 + i * (n + 1); skipped when d_p0 is
+ * NULL), and `shots` samples (d_shots + i * shots): sample s is the first
+ * amplitude index whose cumulative probability exceeds u[i * shots + s]
+ * times the state's total, u in [0, 1) (host array).  One read of each state
+ * builds P0 and a tree of probability sums (groups of 2^10 amplitudes, then
+ * sums of 2^10 entries per level); each sample then reads at most 2^10
+ * values per level.  Asynchronous on `stream`; the host arrays may be reused
+ * when the call returns.  The per-state P0 reduction replaces
+ * statevec_findProbabilityOfZeroKernel (src/cpp/QuEST_gpu_kernel.cu:769-807)
+ * for a batch of registers; sampling has no counterpart in the reference
+ * (measure, src/QuEST_common.c:97-115, draws one qubit at a time). */
+int dq_readout_many(int prec, int n_qubits, const void *const *states, uint32_t count, const double *u,
+                    uint32_t shots, double *d_p0, uint64_t *d_shots, void *d_ws, size_t ws_bytes, void *stream);
+size_t dq_readout_ws_bytes(int n_qubits, uint32_t count, uint32_t shots);
+
+/* Agreement of two n-qubit states a, b of any precisions (e.g. an FP64 and
+ * an FP32 run of one circuit): d_out (device, 5 doubles) = re <a|b>,
+ * im <a|b>, <a|a>, <b|b>, max_i |a_i - b_i|.  Asynchronous on `stream`. */
+int dq_compare_states(const void *a, int prec_a, const void *b, int prec_b, int n_qubits, double *d_out,
+                      void *d_ws, size_t ws_bytes, void *stream);
+size_t dq_compare_ws_bytes(int n_qubits);
+
+/* 0 if all work recorded before `ev` has completed, 1 if not yet (no wait). */
+int dq_event_query(void *ev);
+/* Wait for the work recorded before `ev`. */
+int dq_event_sync(void *ev);
+
 /* Read `count` amplitudes of logical indices start .. start+count-1 as
  * {re, im} doubles into host memory.  l2p[q] is the physical bit that holds
  * logical qubit q (identity if NULL).  Synchronises `stream`.  Replaces
