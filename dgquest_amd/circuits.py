@@ -179,3 +179,60 @@ def grid_random(rows: int, cols: int, depth: int, seed: int) -> List[Gate]:
         for a, b in patterns[cyc % 4]:
             c.append(("controlledPhaseFlip", (a, b), ()))
     return c
+
+
+def hea(n: int, depth: int, seed: int) -> List[Gate]:
+    """Hardware-efficient ansatz: per layer RY(theta) then RZ(phi) on every
+    qubit (angles U[0, 2pi)), then a ring of CNOTs i -> i+1 (and n-1 -> 0)."""
+    rng = MT19937([seed])
+    c: List[Gate] = []
+    twopi = 2.0 * math.pi
+    for _ in range(depth):
+        for q in range(n):
+            c.append(("rotateY", (q,), (twopi * rng.genrand_res53(),)))
+        for q in range(n):
+            c.append(("rotateZ", (q,), (twopi * rng.genrand_res53(),)))
+        if n >= 2:
+            for q in range(n if n > 2 else 1):
+                c.append(("controlledNot", (q, (q + 1) % n), ()))
+    return c
+
+
+def serving_mix(count: int, seed: int = 5, rate: float = 100.0, n_min: int = 10, n_max: int = 30,
+                build: bool = True):
+    """A request stream of the serving workload: Poisson arrivals at `rate`
+    per second; qubit counts log-uniform on [n_min, n_max] (n = floor of
+    exp(U[ln n_min, ln(n_max + 1)))); 40% hardware-efficient ansatz (depth
+    4..16), 30% GHZ + QFT, 30% layered U3 + CZ (depth 8..24).  Returns a list
+    of dicts (kind, n, depth, circuit_seed, shot_seed, arrive_s, circuit);
+    with build=False the circuits are left out (see mix_circuit)."""
+    rng = MT19937([seed])
+    lo, hi = math.log(n_min), math.log(n_max + 1)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += -math.log(1.0 - rng.genrand_res53()) / rate
+        n = min(n_max, int(math.exp(lo + (hi - lo) * rng.genrand_res53())))
+        u = rng.genrand_res53()
+        cseed = rng.genrand_int32()
+        sseed = rng.genrand_int32()
+        if u < 0.4:
+            kind, depth = "hea", 4 + rng.genrand_int32() % 13
+        elif u < 0.7:
+            kind, depth = "ghz_qft", 0
+        else:
+            kind, depth = "u3_cz", 8 + rng.genrand_int32() % 17
+        d = {"kind": kind, "n": n, "depth": depth, "circuit_seed": cseed, "shot_seed": sseed, "arrive_s": t}
+        if build:
+            d["circuit"] = mix_circuit(d)
+        out.append(d)
+    return out
+
+
+def mix_circuit(d) -> List[Gate]:
+    """The circuit of one serving_mix entry."""
+    if d["kind"] == "hea":
+        return hea(d["n"], d["depth"], d["circuit_seed"])
+    if d["kind"] == "ghz_qft":
+        return ghz_qft(d["n"])
+    return layered_u3_cz(d["n"], d["depth"], d["circuit_seed"])

@@ -146,14 +146,15 @@ void Dag::add(const Gate &g) {
         nn.barrier = g.tmask | g.cmask;
     }
     alive_count_++;
-    for (int i = 0; i < nq_; i++) {
-        if (!((nodes_[id].barrier >> i) & 1)) continue;
+    for (uint64_t bm = nodes_[id].barrier; bm; bm &= bm - 1) {
+        const int i = ctz64(bm);
         auto &st = stacks_[i];
         if (!st.empty()) {
             const int top = st.back();
             if (merge_ && nodes_[top].rear.empty() && try_merge(top, nodes_[id])) {
                 nodes_.pop_back();
                 alive_count_--;
+                if (nodes_[top].alive) remerge(top);
                 return;
             }
             nodes_[id].prev.push_back(top);
@@ -162,6 +163,43 @@ void Dag::add(const Gate &g) {
         st.push_back(id);
     }
     if (nodes_[id].prev.empty()) front_.push_back(id);
+}
+
+// A node whose gate just changed (it absorbed a later gate) may now merge
+// into its predecessor, as it would if it were added anew: when that
+// predecessor is its immediate one on every qubit and has no other
+// successor.  Merging eagerly here means the DAG left after a selection
+// needs no rebuild (no pair of remaining gates can newly merge).
+void Dag::remerge(int id) {
+    while (id >= 0) {
+        Node &b = nodes_[id];
+        if (!b.alive || b.prev.empty() || !b.rear.empty()) return;
+        const int a = b.prev[0];
+        for (int p : b.prev)
+            if (p != a) return;
+        Node &an = nodes_[a];
+        for (int r : an.rear)
+            if (r != id) return;
+        if (an.g.tmask != b.g.tmask || an.g.cmask != b.g.cmask) return;
+        // unlink b (it is the top of all its stacks), then offer it to a
+        const std::vector<int> saved_rear = an.rear;
+        for (uint64_t m = b.barrier; m; m &= m - 1) stacks_[ctz64(m)].pop_back();
+        an.rear.clear();
+        b.alive = false;
+        alive_count_--;
+        Node nb;
+        nb.g = b.g;
+        nb.barrier = b.barrier;
+        if (!try_merge(a, nb)) {  // no side effects: restore the link
+            Node &b2 = nodes_[id];
+            b2.alive = true;
+            alive_count_++;
+            nodes_[a].rear = saved_rear;
+            for (uint64_t m = b2.barrier; m; m &= m - 1) stacks_[ctz64(m)].push_back(id);
+            return;
+        }
+        id = nodes_[a].alive ? a : -1;
+    }
 }
 
 void Dag::dfs(int start, uint64_t &mask, int limit) {
@@ -190,7 +228,7 @@ void Dag::dfs(int start, uint64_t &mask, int limit) {
     }
 }
 
-Dag Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates) {
+void Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates) {
     for (auto &nd : nodes_) {
         if (!nd.alive) continue;
         nd.tbits = gate_target_bits(nd.g);
@@ -200,18 +238,34 @@ Dag Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t
     }
     const std::vector<int> front = front_;
     for (int f : front) dfs(f, target_mask, limit);
-    Dag next(nq_, merge_);
+    // The taken gates (the first max_gates selected ones in insertion order)
+    // are closed under predecessors, so on every qubit they form a prefix of
+    // its chain: removing them leaves the links among the others as they are
+    // (and, merges being eager, no two of the others can newly merge).
     size_t taken = 0;
     for (auto &nd : nodes_) {
-        if (!nd.alive) continue;
-        if (nd.selected && taken < max_gates) {
-            out.push_back(nd.g);
-            taken++;
-        } else {
-            next.add(nd.g);
-        }
+        if (!nd.alive || !nd.selected || taken >= max_gates) continue;
+        out.push_back(nd.g);
+        nd.alive = false;
+        nd.prev.clear();
+        nd.rear.clear();
+        taken++;
     }
-    return next;
+    if (!taken) return;
+    alive_count_ -= taken;
+    front_.clear();
+    for (size_t i = 0; i < nodes_.size(); i++) {
+        Node &nd = nodes_[i];
+        if (!nd.alive) continue;
+        nd.prev.erase(std::remove_if(nd.prev.begin(), nd.prev.end(), [&](int p) { return !nodes_[p].alive; }),
+                      nd.prev.end());
+        if (nd.prev.empty()) front_.push_back((int)i);
+    }
+    for (auto &st : stacks_) {
+        size_t k = 0;
+        while (k < st.size() && !nodes_[st[k]].alive) k++;
+        if (k) st.erase(st.begin(), st.begin() + (long)k);
+    }
 }
 
 uint64_t Dag::front_targets() const {

@@ -12,6 +12,11 @@
 //             taken set is closed under predecessors and is emitted in
 //             insertion order, capped at max_gates; everything else goes to a
 //             new DAG (:219-259).
+// The reference builds that new DAG by re-adding the remaining gates, which
+// also merges pairs that became mergeable after their link was made (a gate
+// that absorbed a later gate can now merge into its predecessor).  Here such
+// merges happen at once, when the gate changes, so the selected gates are
+// simply removed in place.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -29,9 +34,9 @@ public:
     bool empty() const { return alive_count_ == 0; }
     size_t size() const { return alive_count_; }
 
-    // Move the selected gates to `out` (appended), grow `target_mask`; the
-    // unselected gates are returned as a new DAG (same merge setting).
-    Dag select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates);
+    // Move the selected gates to `out` (appended) and remove them from this
+    // DAG; grow `target_mask`.
+    void select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates);
 
     // All surviving gates in insertion order (a valid topological order).
     std::vector<Gate> gates() const;
@@ -49,6 +54,7 @@ private:
         bool visited = false, selected = false, alive = true;
     };
     bool try_merge(int top, Node &nn);
+    void remerge(int id);
     void erase_node(int top);
     void dfs(int start, uint64_t &mask, int limit);
 

@@ -362,7 +362,7 @@ void paged_flush(QuregState *s) {
             if (pos[p] < nl) local |= 1ull << p;
         std::vector<Gate> bg;
         uint64_t m = local;
-        Dag nxt = d.select(bg, m, nl, (size_t)-1);
+        d.select(bg, m, nl, (size_t)-1);
         if (!bg.empty()) {
             for (auto &g : bg) {
                 g.tmask = map_mask(g.tmask, pos);
@@ -371,7 +371,6 @@ void paged_flush(QuregState *s) {
             run_section(s, bg, remap, init, ik, iidx, ival);
             remap.clear();
             init = false;
-            d = std::move(nxt);
             continue;
         }
         // stuck: every front gate has a target among the high qubits

@@ -610,7 +610,7 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
         while (!pd.empty()) {
             std::vector<Gate> pg;
             uint64_t pm = 0;
-            pd = pd.select(pg, pm, R, (size_t)-1);
+            pd.select(pg, pm, R, (size_t)-1);
             uint32_t tb = 0;
             for (uint64_t m = pm; m; m &= m - 1) tb |= 1u << c.q2t[ctz64(m)];
             sets.push_back(std::move(pg));
@@ -703,7 +703,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
     while (!d.empty()) {
         std::vector<Gate> bg;
         uint64_t tm = 0x3Full;
-        d = d.select(bg, tm, 6 + t.k, (size_t)t.max_ops);
+        d.select(bg, tm, 6 + t.k, (size_t)t.max_ops);
         bgs.push_back(std::move(bg));
         tms.push_back(tm);
     }
@@ -798,6 +798,114 @@ bool plan_permutation(int n, int prec, const std::vector<int> &src, const std::v
     P.st.sweeps = 1;
     P.st.phases = (int)P.phases.size();
     P.st.exchanges = (int)P.phases.size() - 1;
+    return true;
+}
+
+void append_plan(Plan &dst, const Plan &src) {
+    const uint32_t pb = (uint32_t)dst.phases.size(), ob = (uint32_t)dst.ops.size();
+    for (dq_block b : src.blocks) {
+        b.phase_begin += pb;
+        dst.blocks.push_back(b);
+    }
+    for (dq_phase ph : src.phases) {
+        ph.op_begin += ob;
+        dst.phases.push_back(ph);
+    }
+    dst.ops.insert(dst.ops.end(), src.ops.begin(), src.ops.end());
+    dst.st.sweeps += src.st.sweeps;
+    dst.st.phases += src.st.phases;
+    dst.st.exchanges += src.st.exchanges;
+    dst.st.ops += src.st.ops;
+}
+
+bool plan_restore_mapping(int n, int prec, std::vector<int> &l2p, const TuneParams &tp, std::vector<Plan> &passes,
+                          std::vector<Gate> &gates, std::string &err) {
+    auto bit = [](int q) { return 1ull << q; };
+    std::vector<int> p2l(n);
+    for (int q = 0; q < n; q++) p2l[l2p[q]] = q;
+    auto identity = [&] {
+        for (int q = 0; q < n; q++)
+            if (l2p[q] != q) return false;
+        return true;
+    };
+    if (n <= kSmallMax) {
+        // small registers: textbook SWAPs (3 CNOTs) on the physical positions
+        for (int p = 0; p < n; p++) {
+            const int cur = l2p[p];  // physical position of logical qubit p
+            if (cur == p) continue;
+            for (int k = 0; k < 3; k++) {
+                Gate g;
+                g.type = OP_PAULI_X;
+                g.tmask = (k == 1) ? bit(p) : bit(cur);
+                g.cmask = (k == 1) ? bit(cur) : bit(p);
+                gates.push_back(g);
+            }
+            const int other = p2l[p];
+            p2l[cur] = other;
+            l2p[other] = cur;
+            p2l[p] = p;
+            l2p[p] = p;
+        }
+        return true;
+    }
+    const TuneParams t = fit_shape(tp, n);
+    const int cap_high = t.R + t.W;
+    for (int guard = 0; guard < 4 * n + 4 && !identity(); guard++) {
+        // positions taking part in this pass: pairs (current, home) while the
+        // tile has room for their high qubits (qubits 0..5 are always in it)
+        uint64_t S = 0;
+        int high = 0;
+        for (int lq = 0; lq < n; lq++) {
+            const int cur = l2p[lq];
+            if (cur == lq) continue;
+            const uint64_t add = (bit(cur) | bit(lq)) & ~S;
+            const int ah = popc64(add & ~0x3Full);
+            if (high + ah > cap_high) continue;
+            S |= add;
+            high += ah;
+        }
+        // inside S every qubit whose home is in S goes home; the others fill
+        // the remaining positions
+        std::vector<int> np2l = p2l;
+        uint64_t placed = 0;
+        for (uint64_t m = S; m; m &= m - 1) {
+            const int p = ctz64(m);
+            if ((S >> l2p[p]) & 1) {
+                np2l[p] = p;
+                placed |= bit(p);
+            }
+        }
+        std::vector<int> rest_q, rest_p;
+        for (uint64_t m = S; m; m &= m - 1) {
+            const int lq = p2l[ctz64(m)];
+            if (!((placed >> lq) & 1)) rest_q.push_back(lq);
+        }
+        for (uint64_t m = S & ~placed; m; m &= m - 1) rest_p.push_back(ctz64(m));
+        if (rest_q.size() != rest_p.size()) {
+            err = "internal error in qubit re-placement";
+            return false;
+        }
+        for (size_t i = 0; i < rest_p.size(); i++) np2l[rest_p[i]] = rest_q[i];
+        std::vector<int> src, dst;
+        for (uint64_t m = S; m; m &= m - 1) {
+            const int p = ctz64(m);
+            for (uint64_t m2 = S; m2; m2 &= m2 - 1)
+                if (np2l[ctz64(m2)] == p2l[p] && ctz64(m2) != p) {
+                    src.push_back(p);
+                    dst.push_back(ctz64(m2));
+                }
+        }
+        if (src.empty()) break;
+        Plan P;
+        if (!plan_permutation(n, prec, src, dst, tp, P, err)) return false;
+        passes.push_back(std::move(P));
+        p2l = np2l;
+        for (int p = 0; p < n; p++) l2p[p2l[p]] = p;
+    }
+    if (!identity()) {
+        err = "qubit re-placement did not converge";
+        return false;
+    }
     return true;
 }
 

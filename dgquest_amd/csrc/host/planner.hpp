@@ -63,6 +63,17 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
 bool plan_permutation(int n, int prec, const std::vector<int> &src, const std::vector<int> &dst,
                       const TuneParams &tp, Plan &out, std::string &err);
 
+// Qubit moves that return every logical qubit to its own physical position
+// (l2p[q] = physical position of logical qubit q; the identity afterwards).
+// n > 6: permutation passes, one sweep each, appended to `passes`; n <= 6:
+// SWAPs as CNOT triples on physical positions, appended to `gates` for the
+// caller to plan.  False (with err) if planning fails.
+bool plan_restore_mapping(int n, int prec, std::vector<int> &l2p, const TuneParams &tp, std::vector<Plan> &passes,
+                          std::vector<Gate> &gates, std::string &err);
+
+// Append src's blocks, phases and ops to dst (indices rebased).
+void append_plan(Plan &dst, const Plan &src);
+
 // Clamp a tuning choice to an n-qubit state (tile = 6 + R + W <= n).
 TuneParams fit_shape(const TuneParams &tp, int n);
 

@@ -25,6 +25,7 @@
 #include "dag.hpp"
 #include "paging.hpp"
 #include "runtime.hpp"
+#include "tune.hpp"
 
 using namespace dgq;
 
@@ -183,17 +184,6 @@ inline uint64_t bit(int q) { return 1ull << q; }
 
 void rec_unitary(const Qureg &q, int t, uint64_t cm, const double m[8]) { record(q, OP_UNITARY, bit(t), cm, m); }
 
-TuneParams env_tune(int prec) {
-    TuneParams t;
-    (void)prec;
-    if (const char *e = getenv("DGQ_TUNE")) {
-        int k, R, W, mo;
-        if (sscanf(e, "%d,%d,%d,%d", &k, &R, &W, &mo) == 4) { t.k = k; t.R = R; t.W = W; t.max_ops = mo; }
-    }
-    if (const char *e = getenv("DGQ_FUSE")) t.fuse_diagonals = atoi(e) != 0;
-    return t;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -298,86 +288,22 @@ void ensure_probs(QuregState *s) {
 void materialize_mapping(QuregState *s) {
     flush(s);
     const int n = s->n;
-    auto identity = [&] {
-        for (int q = 0; q < n; q++)
-            if (s->l2p[q] != q) return false;
-        return true;
-    };
-    if (identity()) return;
-    if (n <= 6) {
-        // small registers: undo the relabelling with textbook SWAPs (3 CNOTs)
-        for (int p = 0; p < n; p++) {
-            const int cur = s->l2p[p];  // physical position of logical qubit p
-            if (cur == p) continue;
-            for (int k = 0; k < 3; k++) {
-                Gate g;
-                g.type = OP_PAULI_X;
-                g.tmask = (k == 1) ? bit(p) : bit(cur);
-                g.cmask = (k == 1) ? bit(cur) : bit(p);
-                s->pending.push_back(g);
-            }
-            const int other = s->p2l[p];
-            s->p2l[cur] = other;
-            s->l2p[other] = cur;
-            s->p2l[p] = p;
-            s->l2p[p] = p;
-        }
-        flush(s);
-        return;
+    bool ident = true;
+    for (int q = 0; q < n; q++) ident &= s->l2p[q] == q;
+    if (ident) return;
+    std::vector<Plan> passes;
+    std::vector<Gate> gates;
+    std::string err;
+    if (!plan_restore_mapping(n, s->prec, s->l2p, s->tune, passes, gates, err)) {
+        fprintf(stderr, "dgquest: qubit re-placement failed: %s\n", err.c_str());
+        exit(EXIT_FAILURE);
     }
-    const TuneParams t = fit_shape(s->tune, n);
-    const int cap_high = t.R + t.W;
-    for (int guard = 0; guard < 4 * n + 4 && !identity(); guard++) {
-        // positions taking part in this pass: pairs (current, home) while the
-        // tile has room for their high qubits (qubits 0..5 are always in it)
-        uint64_t S = 0;
-        int high = 0;
-        for (int lq = 0; lq < n; lq++) {
-            const int cur = s->l2p[lq];
-            if (cur == lq) continue;
-            const uint64_t add = (bit(cur) | bit(lq)) & ~S;
-            const int ah = popc64(add & ~0x3Full);
-            if (high + ah > cap_high) continue;
-            S |= add;
-            high += ah;
-        }
-        // inside S every qubit whose home is in S goes home; the others fill
-        // the remaining positions
-        std::vector<int> np2l = s->p2l;
-        uint64_t placed = 0;
-        for (uint64_t m = S; m; m &= m - 1) {
-            const int p = ctz64(m);
-            if ((S >> s->l2p[p]) & 1) { np2l[p] = p; placed |= bit(p); }
-        }
-        std::vector<int> rest_q, rest_p;
-        for (uint64_t m = S; m; m &= m - 1) {
-            const int p = ctz64(m);
-            const int lq = s->p2l[p];
-            if (!((placed >> lq) & 1)) rest_q.push_back(lq);
-        }
-        for (uint64_t m = S & ~placed; m; m &= m - 1) rest_p.push_back(ctz64(m));
-        if (rest_q.size() != rest_p.size()) {
-            fprintf(stderr, "dgquest: internal error in qubit re-placement\n");
-            exit(EXIT_FAILURE);
-        }
-        for (size_t i = 0; i < rest_p.size(); i++) np2l[rest_p[i]] = rest_q[i];
-        std::vector<int> src, dst;
-        for (uint64_t m = S; m; m &= m - 1) {
-            const int p = ctz64(m);
-            for (uint64_t m2 = S; m2; m2 &= m2 - 1)
-                if (np2l[ctz64(m2)] == s->p2l[p] && ctz64(m2) != p) { src.push_back(p); dst.push_back(ctz64(m2)); }
-        }
-        if (src.empty()) break;
-        Plan P;
-        std::string err;
-        if (!plan_permutation(n, s->prec, src, dst, s->tune, P, err)) {
-            fprintf(stderr, "dgquest: permutation planning failed: %s\n", err.c_str());
-            exit(EXIT_FAILURE);
-        }
-        run_plan(s, P);
-        s->p2l = np2l;
-        for (int p = 0; p < n; p++) s->l2p[s->p2l[p]] = p;
-        s->probs_dirty = true;
+    for (const Plan &P : passes) run_plan(s, P);
+    for (int q = 0; q < n; q++) s->p2l[s->l2p[q]] = q;
+    s->probs_dirty = true;
+    if (!gates.empty()) {
+        s->pending.insert(s->pending.end(), gates.begin(), gates.end());
+        flush(s);
     }
 }
 
@@ -432,8 +358,7 @@ Qureg createQureg(int numQubits, QuESTEnv env) {
     QuregState *s = new QuregState();
     s->n = numQubits;
     s->prec = sizeof(qreal) == 8 ? DQ_F64 : DQ_F32;
-    s->tune = env_tune(s->prec);
-    if (env.value) s->tune = env.value->tune_from_table ? env.value->tune : s->tune;
+    s->tune = tune_for(s->prec, numQubits);
     s->l2p.resize(numQubits);
     s->p2l.resize(numQubits);
     for (int i = 0; i < numQubits; i++) s->l2p[i] = s->p2l[i] = i;
@@ -941,7 +866,7 @@ Qureg dgq_create_recorder(int numQubits) {
     QuregState *s = new QuregState();
     s->n = numQubits;
     s->prec = sizeof(qreal) == 8 ? DQ_F64 : DQ_F32;
-    s->tune = env_tune(s->prec);
+    s->tune = tune_for(s->prec, numQubits);
     s->recorder = true;
     s->l2p.resize(numQubits);
     s->p2l.resize(numQubits);
@@ -976,6 +901,63 @@ int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_bl
     if (counts[2]) memcpy(ops, P.ops.data(), P.ops.size() * sizeof(dq_op));
     return 0;
 }
+
+int dgq_recorder_export(Qureg q, dgq_gate_rec *gates, int cap, int *n_gates, int *l2p, int *init_kind,
+                        unsigned long long *init_index, double *init_value) {
+    QuregState *s = q.state;
+    if (!s || !s->recorder || !n_gates) return -1;
+    *n_gates = (int)s->pending.size();
+    if (l2p)
+        for (int i = 0; i < s->n; i++) l2p[i] = s->l2p[i];
+    if (init_kind) *init_kind = s->init_kind;
+    if (init_index) *init_index = s->init_index;
+    if (init_value) *init_value = s->init_value;
+    if ((int)s->pending.size() > cap) return 1;
+    for (size_t i = 0; i < s->pending.size(); i++) {
+        const Gate &g = s->pending[i];
+        dgq_gate_rec &r = gates[i];
+        memset(&r, 0, sizeof r);
+        r.type = g.type;
+        r.tmask = g.tmask;
+        r.cmask = g.cmask;
+        for (int j = 0; j < 8; j++) r.op[j] = g.op[j];
+    }
+    return 0;
+}
+
+int dgq_sample(Qureg q, int shots, const double *u, unsigned long long *out) {
+    QuregState *s = q.state;
+    if (!s || shots < 0 || (shots && (!u || !out))) return -1;
+    if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
+    if (s->paged) engine_fatal("dgq_sample: sampling a paged register", DQ_ERR_UNSUPPORTED);
+    for (int i = 0; i < shots; i++)
+        if (!(u[i] >= 0.0 && u[i] < 1.0)) return -1;
+    if (!shots) return 0;
+    materialize_mapping(s);  // samples walk the CDF in logical index order
+    const size_t res = (((size_t)shots * sizeof(uint64_t)) + 255) / 256 * 256;
+    ensure_ws(s, res + dq_readout_ws_bytes(s->n, 1, (uint32_t)shots));
+    uint64_t *d_out = (uint64_t *)s->d_ws;
+    const void *st = s->d_state;
+    ENG(dq_readout_many(s->prec, s->n, &st, 1, u, (uint32_t)shots, nullptr, d_out, (unsigned char *)s->d_ws + res,
+                        s->ws_cap - res, s->stream));
+    std::vector<uint64_t> idx((size_t)shots);
+    ENG(dq_memcpy_async(idx.data(), d_out, (size_t)shots * sizeof(uint64_t), DQ_D2H, s->stream));
+    ENG(dq_stream_sync(s->stream));
+    for (int i = 0; i < shots; i++) out[i] = idx[i];
+    return 0;
+}
+
+int dgq_tune_lookup(int prec, int n, int out[4]) {
+    if ((prec != DQ_F32 && prec != DQ_F64) || n < 1 || n > 64 || !out) return -1;
+    const TuneParams t = tune_for(prec, n);
+    out[0] = t.k;
+    out[1] = t.R;
+    out[2] = t.W;
+    out[3] = t.max_ops;
+    return 0;
+}
+
+int dgq_tune_reload(const char *path) { return tune_load(path ? path : ""); }
 
 int dgq_paged_info(Qureg q, int *piece_qubits, int *pieces_log2, double stats[10]) {
     QuregState *s = q.state;

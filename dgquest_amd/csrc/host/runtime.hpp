@@ -20,8 +20,6 @@ struct PagedStore;
 
 struct QuESTEnvValue {
     int device = 0;
-    dgq::TuneParams tune;       // default fusion shape for new registers
-    bool tune_from_table = false;
 };
 
 struct QuregState {

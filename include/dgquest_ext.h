@@ -92,6 +92,34 @@ void dgq_destroy_recorder(Qureg q);
 int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_block *blocks, int cap_blocks,
                       dq_phase *phases, int cap_phases, dq_op *ops, int cap_ops, int counts[3], int *l2p);
 
+/* The recording of q as gate records (physical qubit positions, the encoding
+ * of src/cpp/QuEST_backend.h:160-178) for another executor such as the
+ * request server of dgq_serve.h: *n_gates receives the count; gates (cap
+ * entries) the records; l2p (n ints, may be NULL) the logical -> physical
+ * qubit map after them; init_* the pending initialisation (DQ_INIT_*).
+ * Returns 0, 1 when cap is too small (only the counts and map are written),
+ * -1 on bad arguments. */
+int dgq_recorder_export(Qureg q, dgq_gate_rec *gates, int cap, int *n_gates, int *l2p, int *init_kind,
+                        unsigned long long *init_index, double *init_value);
+
+/* `shots` samples of q's state without collapsing it: sample s is the first
+ * basis state (logical index) whose cumulative probability exceeds u[s]
+ * times the total, u[s] in [0, 1).  Flushes recorded gates first.  Returns 0,
+ * or -1 on bad arguments.  Not for paged registers. */
+int dgq_sample(Qureg q, int shots, const double *u, unsigned long long *out);
+
+/* The fused-sweep shape a new register of n qubits and precision prec
+ * (DQ_F32 / DQ_F64) gets: out = {k, R, W, max_ops}.  It comes from
+ * DGQ_TUNE=k,R,W,max_ops when set, else from the tuning table measured by
+ * dgquest_amd/autotune.py (DGQ_TUNE_TABLE, else dgquest_amd/tune_table.txt;
+ * the entry with the nearest n), else the planner defaults.  This replaces
+ * the reference's time predictor (src/cpp/time_predictor.cpp:127-395). */
+int dgq_tune_lookup(int prec, int n, int out[4]);
+
+/* Reload the tuning table from `path` (NULL: the default location).
+ * Returns the number of entries, -1 if the file cannot be read. */
+int dgq_tune_reload(const char *path);
+
 #ifdef __cplusplus
 }
 #endif
