@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-nmax", type=int, default=30)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-dump", type=str, default="", help="write per-request results (JSON lines) here")
     return ap.parse_args()
 
 
@@ -184,6 +185,14 @@ def serving(q64, args):
         recs2 = [S.record(q64, d["n"], circuits.mix_circuit(d)) for d in mix2]
         rs2 = S.RequestSet(recs2, [1024] * len(mix2), [d["shot_seed"] for d in mix2])
         at_load = srv.replay(rs2, [d["arrive_s"] for d in mix2])
+    if args.serve_dump:
+        with open(args.serve_dump, "w") as f:
+            for tag, rows, mx in (("plain", plain, mix), ("served", served, mix), ("at_load", at_load, mix2)):
+                for d, r in zip(mx, rows):
+                    f.write(json.dumps({"run": tag, "kind": d["kind"], "n": d["n"], "depth": d["depth"],
+                                        **{k: r[k] for k in ("status", "t_arrive_ms", "t_start_ms", "t_done_ms",
+                                                             "batch_size", "rechecked", "fidelity", "sum_z",
+                                                             "shot_mismatches")}}) + "\n")
     bad = [r for r in plain + served + at_load if r["status"] != 0]
     if bad:
         raise RuntimeError(f"{len(bad)} requests failed: {bad[0].get('error')}")

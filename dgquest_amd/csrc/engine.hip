@@ -236,6 +236,20 @@ __device__ __forceinline__ void load_tile(Regs<T, R> &a, const dq_block *__restr
         }                                                                                 \
     } while (0)
 
+// A one-phase block that moves a qubit between wave bits (ld_wave != st_wave)
+// stores part of each wave's amplitudes where another wave of the workgroup
+// loaded: every load of the tile must be done before the first store.  With
+// two or more phases the first LDS exchange already orders them (each wave
+// writes its loaded tile to LDS before the exchange barrier).
+template <int W>
+__device__ __forceinline__ bool store_needs_barrier(const dq_block *__restrict__ blk) {
+    if (blk->phase_count != 1) return false;
+    bool x = false;
+#pragma unroll
+    for (int w = 0; w < W; w++) x |= blk->ld_wave[w] != blk->st_wave[w];
+    return x;
+}
+
 // The phases of a block on one tile: exchanges through LDS between phases,
 // the op program of each phase in the assembly interpreter.  lead: a barrier
 // before the first exchange (LDS may still be read for a previous tile).
@@ -267,6 +281,7 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
     load_tile<T, R, W>(a, blk, st, base, wv, lane);
     process_tile<T, R, W, SPLIT>(a, lds_raw, phases + blk->phase_begin, blk->phase_count,
                                  pprog + 2 * blk->phase_begin, prog, base | job.base_hi, tid, false);
+    if (store_needs_barrier<W>(blk)) __syncthreads();
     DQ_STORE_TILE(a, base);
 }
 
@@ -305,6 +320,7 @@ sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ 
     const uint32_t nph = blk->phase_count;
     const uint32_t *__restrict__ pp = pprog + 2 * blk->phase_begin;
     const uint64_t hi = job.base_hi;
+    const bool wave_perm = store_needs_barrier<W>(blk);
     uint64_t t = blockIdx.x;
     if (t >= ntiles) return;
     Regs<T, R> a, b;
@@ -320,6 +336,7 @@ sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ 
         const uint64_t bb = tile_base(msk, t1 < ntiles ? t1 : ntiles - 1);
         load_tile<T, R, W>(b, blk, st, bb, wv, lane);
         process_tile<T, R, W, SPLIT>(a, lds_raw, ph, nph, pp, prog, ba | hi, tid, lead);
+        if (wave_perm) __syncthreads();
         DQ_STORE_TILE(a, ba);
         lead = true;
         if (t1 >= ntiles) break;
@@ -327,6 +344,7 @@ sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ 
         ba = tile_base(msk, t2 < ntiles ? t2 : ntiles - 1);
         load_tile<T, R, W>(a, blk, st, ba, wv, lane);
         process_tile<T, R, W, SPLIT>(b, lds_raw, ph, nph, pp, prog, bb | hi, tid, lead);
+        if (wave_perm) __syncthreads();
         DQ_STORE_TILE(b, bb);
         if (t2 >= ntiles) break;
         t = t2;
