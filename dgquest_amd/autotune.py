@@ -21,7 +21,8 @@ import time
 # (k, R, W): tile = 6 + R + W qubits, k = target qubits above the six coalesced
 # ones per sweep; only shapes with built kernels (engine.hip SweepTable)
 CANDIDATES = [(3, 3, 0), (4, 4, 0), (5, 5, 0), (5, 4, 1), (6, 5, 1), (6, 4, 2), (7, 5, 2), (6, 3, 3),
-              (7, 4, 3)]
+              (7, 4, 3), (8, 5, 3)]
+FP32_ONLY = {(8, 5, 3)}
 
 
 def measure(quest, eng, env, n, shape, circ, reps):
@@ -59,7 +60,7 @@ def tune(sizes, precs=(2, 1), depth=8, reps=2, log=sys.stderr):
             circ = circuits.layered_u3_cz(n, depth, 1)
             res = {}
             for shape in CANDIDATES:
-                if 6 + shape[1] + shape[2] > n:
+                if 6 + shape[1] + shape[2] > n or (prec == 2 and shape in FP32_ONLY):
                     continue
                 res[shape] = measure(quest, eng, env, n, shape, circ, reps)
             if not res:

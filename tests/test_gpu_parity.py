@@ -250,3 +250,21 @@ def test_30q_layered_norm_and_inverse(quest, env):
     assert abs(a0.real - 1.0) <= 1e-12 and abs(a0.imag) <= 1e-12
     assert abs(quest.calcProbOfOutcome(q, 29, 0) - 1.0) <= 1e-12
     quest.destroyQureg(q, env)
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 0, 4096), (6, 4, 2, 4096), (7, 5, 2, 4096), (7, 4, 3, 4096),
+                                   (6, 3, 3, 4096), (8, 5, 3, 4096)])
+def test_tile_shapes_fp32(quest32, env32, oracle, shape):
+    """FP32 kernels (split exchanges, and the FP32-only 14-bit tile) on a
+    17-qubit circuit: max |delta amplitude| <= 1e-5 and fidelity >= 1 - 1e-6
+    against the FP64 oracle."""
+    n = 17
+    rng = np.random.default_rng(78)
+    circ = H.random_circuit(rng, n, 150)
+    q, sv, st = run_gpu(quest32, env32, n, circ, tune=shape)
+    ref = run_oracle(oracle, n, circ)
+    err = np.abs(sv - ref).max()
+    fid = abs(np.vdot(ref, sv)) ** 2 / np.vdot(sv, sv).real
+    print(f"fp32 shape={shape} err={err:.3e} fid={fid:.9f} stats={st}", flush=True)
+    assert err <= 1e-5 and fid >= 1 - 1e-6
+    quest32.destroyQureg(q, env32)

@@ -147,3 +147,20 @@ def test_tune_table_lookup(q64, tmp_path):
         assert look(2, 30) == (6, 4, 2, 4096)  # empty table: planner defaults
     finally:
         lib.dgq_tune_reload(None)
+
+
+def test_both_precisions_in_one_process():
+    """Both API libraries loaded in one process (FP64 first): calls between API
+    functions inside the FP32 library (fSim -> controlledPhaseShift with a
+    float angle, ...) must stay inside it, so both record the same gates."""
+    from dgquest_amd import quest as Q
+
+    q64, q32 = Q.load(2), Q.load(1)
+    circ = [("fSim", (0, 1), (0.3, 0.2)), ("rotateAroundAxis", (2,), (0.7, 0.1, 0.2, 0.3)),
+            ("controlledRotateAroundAxis", (0, 2), (0.4, 0.3, -0.2, 0.5)), ("u3Gate", (1,), (0.1, 0.2, 0.3))]
+    r64 = S.record(q64, 3, circ)
+    r32 = S.record(q32, 3, circ)
+    assert len(r64.gates) == len(r32.gates)
+    for a, b in zip(r64.gates, r32.gates):
+        assert a["type"] == b["type"] and a["tmask"] == b["tmask"] and a["cmask"] == b["cmask"]
+        assert np.abs(a["op"] - b["op"]).max() < 1e-6  # float arguments, double records
