@@ -17,6 +17,13 @@ roofline: the sweep kernel's algorithmic bytes per launch over its average
 launch duration, timed with HIP events on the register's own stream.
 cpu_baseline: the CPU oracle (oracle/oracle.c, OpenMP, gate by gate) on a
 bounded sample of the same circuit on this host.
+
+serving: a Poisson stream of circuits on 10..30 qubits served in FP32 by the
+request server (continuous batching under a 200 GB HBM budget, 1% FP64
+recheck) against the plain one-at-a-time FP64 loop on the same arrivals:
+circuits/s of both, their ratio, p50/p99 latency, and latency at half the
+served capacity.  paged_bv: a Bernstein-Vazirani register forced into pinned
+host pieces by a small device budget.
 """
 import argparse
 import json
