@@ -236,3 +236,34 @@ def mix_circuit(d) -> List[Gate]:
     if d["kind"] == "ghz_qft":
         return ghz_qft(d["n"])
     return layered_u3_cz(d["n"], d["depth"], d["circuit_seed"])
+
+
+def grid_circuit(rows: int, cols: int, depth: int, seed: int) -> List[Gate]:
+    """Random circuit on a rows x cols grid (qubit = r * cols + c): per cycle a
+    sqrt(X), sqrt(Y) or sqrt(W) on every qubit, drawn uniformly from the seed
+    and never the same gate twice in a row on a qubit; then CZ on one of four
+    nearest-neighbour edge patterns in rotation (cycle i uses pattern i % 4:
+    horizontal pairs from even columns, from odd columns, vertical pairs from
+    even rows, from odd rows)."""
+    rng = MT19937([seed])
+    names = ("SqX", "SqY", "SqW")
+    last = [-1] * (rows * cols)
+    c: List[Gate] = []
+    for cyc in range(depth):
+        for q in range(rows * cols):
+            if last[q] < 0:
+                g = rng.genrand_int32() % 3
+            else:
+                g = (last[q] + 1 + rng.genrand_int32() % 2) % 3
+            last[q] = g
+            c.append((names[g], (q,), ()))
+        pat = cyc % 4
+        if pat < 2:
+            for r in range(rows):
+                for col in range(pat, cols - 1, 2):
+                    c.append(("controlledPhaseFlip", (r * cols + col, r * cols + col + 1), ()))
+        else:
+            for r in range(pat - 2, rows - 1, 2):
+                for col in range(cols):
+                    c.append(("controlledPhaseFlip", (r * cols + col, (r + 1) * cols + col), ()))
+    return c

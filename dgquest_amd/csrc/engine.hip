@@ -782,6 +782,144 @@ compare_stage1(const typename V2<TA>::type *__restrict__ a, const typename V2<TB
     }
 }
 
+// ----------------------------------------------------------------------------
+// dense sweeps: FP32 tile in LDS, dense 2^k x 2^k complex blocks on MFMA
+// ----------------------------------------------------------------------------
+// A dense op on k tile bits is one real GEMM per tile:
+//   [Re y; Im y] = [[Ur, -Ui], [Ui, Ur]] [Re x; Im x],
+// rows and the contraction index both of size M = 2^(k+1), one column per
+// assignment of the other T - k tile bits (NC = 2^(T-k) columns).
+// v_mfma_f32_32x32x2f32: lane l supplies A[l % 32][l / 32] and
+// B[l / 32][l % 32]; result register v of lane l is
+// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].  The host lays every real
+// matrix out so that the A operand of one MFMA is 64 consecutive floats:
+//   mat[(rb * M/2 + p) * 64 + l] = A[32 rb + l % 32][2 p + l / 32].
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+#define DENSE_T_MAX 13
+#define DENSE_THREADS 256
+
+// LDS slot of tile element e: the low 5 bits are xored with bits 5..9 and
+// 10..14, spreading power-of-two strides over the 32 banks (a bijection:
+// only the low bits change, by a function of the others).
+__device__ __forceinline__ uint32_t dswz(uint32_t e) { return e ^ ((e >> 5) & 31u) ^ ((e >> 10) & 31u); }
+
+__device__ __forceinline__ uint32_t deposit_bits(uint32_t v, const uint32_t *pos, int cnt) {
+    uint32_t r = 0;
+    for (int i = 0; i < cnt; i++) r |= ((v >> i) & 1u) << pos[i];
+    return r;
+}
+
+template <int KL>
+__device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restrict__ im, uint32_t T,
+                                         const dq_dense_op &op, const float *__restrict__ mat,
+                                         uint32_t *__restrict__ rowoff, uint32_t *__restrict__ coloff, uint32_t tid) {
+    constexpr int K = 1 << KL, M = 2 * K, MB = M / 32;
+    constexpr int RB = MB < 4 ? MB : 4;          // row blocks per work item
+    constexpr int IMAX = KL == 5 ? 2 : 1;        // work items per wave at T = 13
+    const uint32_t NC = 1u << (T - KL);
+    const uint32_t NCH = NC / 32;
+    const uint32_t items = NCH * (MB / RB);
+    const uint32_t w = tid >> 6, lane = tid & 63;
+    // index tables of this op: matrix index -> tile offset, column -> tile offset
+    uint32_t obits[8], cbits[DENSE_T_MAX];
+    uint32_t used = 0;
+    for (int j = 0; j < KL; j++) { obits[j] = op.bits[j]; used |= 1u << op.bits[j]; }
+    int nc = 0;
+    for (uint32_t b = 0; b < T; b++)
+        if (!((used >> b) & 1u)) cbits[nc++] = b;
+    __syncthreads();  // the previous op's writes and table reads are done
+    for (uint32_t j = tid; j < (uint32_t)K; j += DENSE_THREADS) rowoff[j] = deposit_bits(j, obits, KL);
+    for (uint32_t c = tid; c < NC; c += DENSE_THREADS) coloff[c] = deposit_bits(c, cbits, nc);
+    __syncthreads();
+    v16f acc[IMAX][RB];
+#pragma unroll
+    for (int i = 0; i < IMAX; i++)
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+            for (int v = 0; v < 16; v++) acc[i][r][v] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < IMAX; i++) {
+        const uint32_t item = w + 4u * i;
+        if (item >= items) break;  // uniform over the wave
+        const uint32_t ch = item % NCH, rg = item / NCH;
+        const uint32_t col = ch * 32 + (lane & 31u);
+        const uint32_t cofs = coloff[col];
+        const float *__restrict__ ma = mat + (size_t)(rg * RB) * (M / 2) * 64 + lane;
+#pragma unroll 4
+        for (int p = 0; p < M / 2; p++) {
+            const uint32_t row = 2 * p + (lane >> 5);
+            const uint32_t e = dswz(rowoff[row & (K - 1)] | cofs);
+            const float b = (row >> KL) ? im[e] : re[e];
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                const float a = ma[((size_t)r * (M / 2) + p) * 64];
+                acc[i][r] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[i][r], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();  // every wave has read the whole tile: write in place
+#pragma unroll
+    for (int i = 0; i < IMAX; i++) {
+        const uint32_t item = w + 4u * i;
+        if (item >= items) break;
+        const uint32_t ch = item % NCH, rg = item / NCH;
+        const uint32_t cofs = coloff[ch * 32 + (lane & 31u)];
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const uint32_t row = (rg * RB + r) * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const uint32_t e = dswz(rowoff[row & (K - 1)] | cofs);
+                if (row >> KL) im[e] = acc[i][r][v];
+                else re[e] = acc[i][r][v];
+            }
+    }
+}
+
+// One dense block over every tile of an FP32 state: grid = tiles, 256 threads.
+template <int KL>
+__global__ void __launch_bounds__(DENSE_THREADS)
+dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restrict__ ops, const float *__restrict__ mats,
+             typename V2<float>::type *__restrict__ st) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const uint32_t T = blk->tile_bits;
+    const uint32_t NT = 1u << T;
+    float *re = reinterpret_cast<float *>(lds_raw);
+    float *im = re + NT;
+    uint32_t *rowoff = reinterpret_cast<uint32_t *>(im + NT);
+    uint32_t *coloff = rowoff + 128;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t base = tile_base(blk->nontile_mask, blockIdx.x);
+    uint64_t hq[DENSE_T_MAX];
+#pragma unroll
+    for (int b = 0; b < DENSE_T_MAX; b++) hq[b] = 1ull << blk->tile_q[b];
+    // load: element e of the tile -> qubits of its bits (bits 0..5 are qubits 0..5)
+    for (uint32_t e = tid; e < NT; e += DENSE_THREADS) {
+        uint64_t g = base | (e & 63u);
+        for (uint32_t b = 6; b < T; b++)
+            if ((e >> b) & 1u) g |= hq[b];
+        const typename V2<float>::type v = __builtin_nontemporal_load(st + g);
+        re[dswz(e)] = v.x;
+        im[dswz(e)] = v.y;
+    }
+    const size_t msz = (size_t)(2u << KL) * (2u << KL);
+    for (uint32_t o = 0; o < blk->op_count; o++) {
+        const dq_dense_op op = ops[blk->op_begin + o];
+        dense_op<KL>(re, im, T, op, mats + op.mat * msz, rowoff, coloff, tid);
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < NT; e += DENSE_THREADS) {
+        uint64_t g = base | (e & 63u);
+        for (uint32_t b = 6; b < T; b++)
+            if ((e >> b) & 1u) g |= hq[b];
+        typename V2<float>::type v;
+        v.x = re[dswz(e)];
+        v.y = im[dswz(e)];
+        __builtin_nontemporal_store(v, st + g);
+    }
+}
+
 // one workgroup: out = {re <a|b>, im <a|b>, <a|a>, <b|b>, max |a_i - b_i|}
 __global__ void __launch_bounds__(256) compare_stage2(const double *__restrict__ part, uint32_t ngroups, double *__restrict__ out) {
     __shared__ double red[4][CMP_SLOTS];
@@ -992,6 +1130,9 @@ extern "C" int dq_device_init(int device) {
         }
         HIPC(hipFuncSetAttribute((const void *)small_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
         HIPC(hipFuncSetAttribute((const void *)small_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+        HIPC(hipFuncSetAttribute((const void *)dense_kernel<5>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+        HIPC(hipFuncSetAttribute((const void *)dense_kernel<6>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+        HIPC(hipFuncSetAttribute((const void *)dense_kernel<7>, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
         g_device = device;
     }
     return DQ_OK;
@@ -1615,6 +1756,93 @@ extern "C" int dq_compare_states(const void *a, int prec_a, const void *b, int p
     HIPC(hipGetLastError());
     hipLaunchKernelGGL(compare_stage2, dim3(1), dim3(256), 0, s, part, (uint32_t)groups, d_out);
     HIPC(hipGetLastError());
+    return DQ_OK;
+}
+
+// ---- dense sweeps -------------------------------------------------------------
+namespace {
+size_t dense_mat_floats(int k) { return (size_t)(2u << k) * (2u << k); }
+size_t dense_lds_bytes(uint32_t T) { return 2 * ((size_t)4 << T) + (128 + 256) * sizeof(uint32_t); }
+}  // namespace
+
+extern "C" size_t dq_dense_ws_bytes(int k, uint32_t n_blocks, uint32_t n_ops, uint32_t n_mats) {
+    if (k < 5 || k > 7) return 0;
+    return align_up(n_blocks * sizeof(dq_dense_block), 256) + align_up(n_ops * sizeof(dq_dense_op), 256) +
+           align_up((size_t)n_mats * dense_mat_floats(k) * sizeof(float), 256);
+}
+
+extern "C" int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_blocks, const dq_dense_op *ops,
+                              uint32_t n_ops, const double *mats, uint32_t n_mats, void *state, void *d_ws,
+                              size_t ws_bytes, void *stream) {
+    if (k < 5 || k > 7) return fail(DQ_ERR_ARG, "dq_apply_dense: k must be 5, 6 or 7");
+    if (!blocks || !state || !d_ws || (n_ops && !ops) || (n_mats && !mats))
+        return fail(DQ_ERR_ARG, "dq_apply_dense: null argument");
+    const size_t need = dq_dense_ws_bytes(k, n_blocks, n_ops, n_mats);
+    if (ws_bytes < need) return fail(DQ_ERR_ARG, "dq_apply_dense: workspace %zu < %zu bytes", ws_bytes, need);
+    for (uint32_t i = 0; i < n_blocks; i++) {
+        const dq_dense_block &b = blocks[i];
+        const uint32_t n = b.n_qubits, T = b.tile_bits;
+        if (n < 6 || n > 40 || T > DENSE_T_MAX || T > n || (int)T < k + 5)
+            return fail(DQ_ERR_ARG, "dense block %u: need k + 5 <= T <= min(n, 13) (n=%u T=%u k=%d)", i, n, T, k);
+        uint64_t m = 0;
+        for (uint32_t t = 0; t < T; t++) {
+            if (t < 6 && b.tile_q[t] != t) return fail(DQ_ERR_ARG, "dense block %u: tile bits 0..5 must be qubits 0..5", i);
+            if (b.tile_q[t] >= n || ((m >> b.tile_q[t]) & 1)) return fail(DQ_ERR_ARG, "dense block %u: bad tile qubits", i);
+            m |= 1ull << b.tile_q[t];
+        }
+        if ((m | b.nontile_mask) != ((n == 64) ? ~0ull : ((1ull << n) - 1)) || (m & b.nontile_mask))
+            return fail(DQ_ERR_ARG, "dense block %u: tile and non-tile masks inconsistent", i);
+        if ((uint64_t)b.op_begin + b.op_count > n_ops) return fail(DQ_ERR_ARG, "dense block %u: op range", i);
+        for (uint32_t o = b.op_begin; o < b.op_begin + b.op_count; o++) {
+            uint32_t u = 0;
+            for (int j = 0; j < k; j++) {
+                if (ops[o].bits[j] >= T || ((u >> ops[o].bits[j]) & 1)) return fail(DQ_ERR_ARG, "dense op %u: bad bits", o);
+                u |= 1u << ops[o].bits[j];
+            }
+            if (ops[o].mat >= n_mats) return fail(DQ_ERR_ARG, "dense op %u: matrix index", o);
+        }
+        if (n - T > 31) return fail(DQ_ERR_UNSUPPORTED, "dense block %u: grid too large", i);
+    }
+    // upload: blocks | ops | real matrices in MFMA operand order
+    const size_t ob = align_up(n_blocks * sizeof(dq_dense_block), 256);
+    const size_t mo = ob + align_up(n_ops * sizeof(dq_dense_op), 256);
+    const int K = 1 << k, M = 2 * K;
+    std::vector<unsigned char> buf(mo + (size_t)n_mats * dense_mat_floats(k) * sizeof(float), 0);
+    memcpy(buf.data(), blocks, n_blocks * sizeof(dq_dense_block));
+    if (n_ops) memcpy(buf.data() + ob, ops, n_ops * sizeof(dq_dense_op));
+    float *dst = reinterpret_cast<float *>(buf.data() + mo);
+    for (uint32_t mi = 0; mi < n_mats; mi++) {
+        const double *U = mats + (size_t)mi * K * K * 2;
+        float *D = dst + (size_t)mi * dense_mat_floats(k);
+        for (int rb = 0; rb < M / 32; rb++)
+            for (int p = 0; p < M / 2; p++)
+                for (int l = 0; l < 64; l++) {
+                    const int r = rb * 32 + (l & 31), c = 2 * p + (l >> 5);
+                    const int ur = r & (K - 1), uc = c & (K - 1);
+                    const double Ure = U[2 * ((size_t)ur * K + uc)], Uim = U[2 * ((size_t)ur * K + uc) + 1];
+                    double a;
+                    if (r < K) a = c < K ? Ure : -Uim;
+                    else a = c < K ? Uim : Ure;
+                    D[((size_t)rb * (M / 2) + p) * 64 + l] = (float)a;
+                }
+    }
+    int rc = stage_upload(d_ws, buf, stream);
+    if (rc) return rc;
+    unsigned char *d = (unsigned char *)d_ws;
+    const dq_dense_block *d_blocks = (const dq_dense_block *)d;
+    const dq_dense_op *d_ops = (const dq_dense_op *)(d + ob);
+    const float *d_mats = (const float *)(d + mo);
+    hipStream_t s = (hipStream_t)stream;
+    for (uint32_t i = 0; i < n_blocks; i++) {
+        const dq_dense_block &b = blocks[i];
+        const unsigned grid = 1u << (b.n_qubits - b.tile_bits);
+        const size_t lds = dense_lds_bytes(b.tile_bits);
+        auto *st = (typename V2<float>::type *)state;
+        if (k == 5) hipLaunchKernelGGL(dense_kernel<5>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, st);
+        else if (k == 6) hipLaunchKernelGGL(dense_kernel<6>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, st);
+        else hipLaunchKernelGGL(dense_kernel<7>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, st);
+        HIPC(hipGetLastError());
+    }
     return DQ_OK;
 }
 

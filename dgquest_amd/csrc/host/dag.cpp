@@ -228,10 +228,10 @@ void Dag::dfs(int start, uint64_t &mask, int limit) {
     }
 }
 
-void Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates) {
+void Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates, bool full_support) {
     for (auto &nd : nodes_) {
         if (!nd.alive) continue;
-        nd.tbits = gate_target_bits(nd.g);
+        nd.tbits = full_support ? nd.barrier : gate_target_bits(nd.g);
         nd.remain = nd.prev.size();
         nd.visited = false;
         nd.selected = false;

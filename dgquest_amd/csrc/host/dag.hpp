@@ -35,8 +35,11 @@ public:
     size_t size() const { return alive_count_; }
 
     // Move the selected gates to `out` (appended) and remove them from this
-    // DAG; grow `target_mask`.
-    void select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates);
+    // DAG; grow `target_mask`.  full_support counts every qubit a gate
+    // touches (controls and diagonal gates too), for dense blocks that form
+    // one matrix over their qubits.
+    void select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates,
+                bool full_support = false);
 
     // All surviving gates in insertion order (a valid topological order).
     std::vector<Gate> gates() const;

@@ -26,6 +26,7 @@
 #include "paging.hpp"
 #include "runtime.hpp"
 #include "tune.hpp"
+#include "dense.hpp"
 
 using namespace dgq;
 
@@ -250,11 +251,33 @@ void flush(QuregState *s) {
     }
     const auto t0 = std::chrono::steady_clock::now();
     const unsigned long long state_bytes = (unsigned long long)(s->prec == DQ_F64 ? 16 : 8) << s->n;
+    s->last_dense_flops = 0.0;
     if (s->pending.empty()) {
         ENG(dq_init_state(s->d_state, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->stream));
         s->last_stats = PlanStats();
         s->last_bytes = state_bytes;
         s->last_launches = 1;
+    } else if (s->dense_k) {
+        // precision form: the initial state is written first, then dense sweeps
+        s->last_bytes = 0;
+        if (s->init_kind != DQ_INIT_LOAD) {
+            ENG(dq_init_state(s->d_state, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->stream));
+            s->last_bytes += state_bytes;
+        }
+        DensePlan D;
+        std::string err;
+        if (!plan_dense(s->pending, s->n, s->dense_k, D, err)) engine_fatal(("dense planning: " + err).c_str(), DQ_ERR_ARG);
+        const uint32_t nb = (uint32_t)D.blocks.size(), no = (uint32_t)D.ops.size();
+        ensure_ws(s, dq_dense_ws_bytes(s->dense_k, nb, no, no));
+        ENG(dq_apply_dense(s->dense_k, D.blocks.data(), nb, D.ops.data(), no, D.mats.data(), no, s->d_state, s->d_ws,
+                           s->ws_cap, s->stream));
+        s->last_stats = PlanStats();
+        s->last_stats.sweeps = (int)nb;
+        s->last_stats.ops = (int)no;
+        s->last_stats.gates_in = (int)s->pending.size();
+        s->last_launches = (int)nb;
+        s->last_bytes += 2 * state_bytes * nb;
+        s->last_dense_flops = D.flops;
     } else {
         Plan P;
         plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P);
@@ -958,6 +981,44 @@ int dgq_tune_lookup(int prec, int n, int out[4]) {
 }
 
 int dgq_tune_reload(const char *path) { return tune_load(path ? path : ""); }
+
+int dgq_set_dense(Qureg q, int k) {
+    QuregState *s = q.state;
+    if (!s || k < 0) return -1;
+    if (k == 0) {
+        s->dense_k = 0;
+        return 0;
+    }
+    const int T = s->n < 13 ? s->n : 13;
+    if (s->prec != DQ_F32 || s->paged || k < 5 || k > 7 || T < k + 5) return -1;
+    flush(s);  // gates recorded so far run in the current form
+    s->dense_k = k;
+    return 0;
+}
+
+int dgq_last_dense_stats(Qureg q, double out[4]) {
+    QuregState *s = q.state;
+    if (!s || !out) return -1;
+    out[0] = s->last_stats.sweeps;
+    out[1] = s->last_stats.ops;
+    out[2] = s->last_dense_flops;
+    out[3] = s->last_flush_ms;
+    return 0;
+}
+
+int dgq_recorder_plan_dense(Qureg q, int k, int cap_ops, int *n_ops, int *n_blocks, int *op_qubits, double *mats) {
+    QuregState *s = q.state;
+    if (!s || !s->recorder || !n_ops) return -1;
+    DensePlan D;
+    std::string err;
+    if (!plan_dense(s->pending, s->n, k, D, err)) return -1;
+    *n_ops = (int)D.ops.size();
+    if (n_blocks) *n_blocks = (int)D.blocks.size();
+    if ((int)D.ops.size() > cap_ops) return 1;
+    if (op_qubits) memcpy(op_qubits, D.op_qubits.data(), D.op_qubits.size() * sizeof(int));
+    if (mats) memcpy(mats, D.mats.data(), D.mats.size() * sizeof(double));
+    return 0;
+}
 
 int dgq_paged_info(Qureg q, int *piece_qubits, int *pieces_log2, double stats[10]) {
     QuregState *s = q.state;

@@ -42,6 +42,8 @@ struct QuregState {
     int last_launches = 0;
     double last_flush_ms = 0.0;
     bool recorder = false;            // records and plans only, no device state
+    int dense_k = 0;                  // > 0: flushes run as dense sweeps of this width (FP32)
+    double last_dense_flops = 0.0;    // device GEMM flops of the last dense flush
     dgq::PagedStore *paged = nullptr; // state in pinned host pieces, paged through the device
 };
 

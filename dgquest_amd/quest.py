@@ -172,6 +172,14 @@ class QuEST:
                                         C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
 
+        L.dgq_set_dense.restype = C.c_int
+        L.dgq_set_dense.argtypes = [Qureg, C.c_int]
+        L.dgq_last_dense_stats.restype = C.c_int
+        L.dgq_last_dense_stats.argtypes = [Qureg, C.POINTER(C.c_double)]
+        L.dgq_recorder_plan_dense.restype = C.c_int
+        L.dgq_recorder_plan_dense.argtypes = [Qureg, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                              C.c_void_p, C.c_void_p]
+
     # --- thin pass-throughs -------------------------------------------------
     def __getattr__(self, name):
         return getattr(self.lib, name)
@@ -261,6 +269,30 @@ class QuEST:
         if rc != 0:
             raise RuntimeError("dgq_recorder_plan failed")
         return b, p, o, list(l2p)
+
+    def dense_stats(self, q: Qureg):
+        """Of the last flush: sweeps, dense ops, device GEMM flops, host ms."""
+        v = (C.c_double * 4)()
+        self.lib.dgq_last_dense_stats(q, v)
+        return {"sweeps": int(v[0]), "ops": int(v[1]), "flops": v[2], "flush_ms": v[3]}
+
+    def recorder_plan_dense(self, q: Qureg, k: int):
+        """Dense plan of a recording register: (op qubits [ops x k], matrices [ops x 2^k x 2^k] complex)."""
+        import numpy as np
+
+        n_ops = C.c_int()
+        n_blocks = C.c_int()
+        rc = self.lib.dgq_recorder_plan_dense(q, k, 0, C.byref(n_ops), C.byref(n_blocks), None, None)
+        if rc < 0:
+            raise RuntimeError("dgq_recorder_plan_dense failed")
+        K = 1 << k
+        qs = np.zeros(n_ops.value * k, dtype=np.int32)
+        mats = np.zeros(n_ops.value * K * K * 2, dtype=np.float64)
+        rc = self.lib.dgq_recorder_plan_dense(q, k, n_ops.value, C.byref(n_ops), C.byref(n_blocks), qs.ctypes.data,
+                                              mats.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("dgq_recorder_plan_dense failed")
+        return qs.reshape(-1, k), mats.view(np.complex128).reshape(-1, K, K), n_blocks.value
 
     def plan_stats(self, q: Qureg):
         """Statistics of the last flush: sweeps, phases, exchanges, ops, gates_in, k, R, W + flush ms."""

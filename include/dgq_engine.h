@@ -204,6 +204,36 @@ size_t dq_prob_ws_bytes(int n_qubits);
 int dq_prob_zero_all_async(const void *state, int n_qubits, int prec, double *d_out,
                            void *d_ws, size_t ws_bytes, void *stream);
 
+/* ---- dense sweeps: FP32 state, dense 2^k x 2^k blocks on MFMA --------------
+ * A dense block loads a tile of T qubits (qubits 0..5 always tile bits
+ * 0..5; tile bit b = qubit tile_q[b]) into LDS, applies its dense ops in
+ * order, and stores the tile back.  A dense op multiplies, for every
+ * assignment of the other tile bits, the 2^k-vector of amplitudes indexed by
+ * its k tile bits (matrix-index bit j = tile bit bits[j]) by a 2^k x 2^k
+ * complex matrix, computed as one real GEMM per tile on FP32 MFMA with FP32
+ * accumulation.  k is 5, 6 or 7, the same for all ops of a call, and
+ * k + 5 <= T <= 13. */
+typedef struct dq_dense_block {
+    uint32_t n_qubits;
+    uint32_t tile_bits;   /* T */
+    uint32_t op_begin;    /* index into the op array of the call */
+    uint32_t op_count;
+    uint64_t nontile_mask;
+    uint8_t tile_q[16];
+} dq_dense_block;
+
+typedef struct dq_dense_op {
+    uint32_t bits[8];     /* tile bits of matrix-index bits 0..k-1 */
+    uint64_t mat;         /* index of the op's matrix in the call's matrix array */
+} dq_dense_op;
+
+/* Apply `n_blocks` dense blocks in order to an FP32 state of n qubits.
+ * mats: n_mats complex matrices, 2^k x 2^k each, row-major {re, im} doubles
+ * (host memory; rounded to float and laid out for MFMA by this call). */
+int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_blocks, const dq_dense_op *ops, uint32_t n_ops,
+                   const double *mats, uint32_t n_mats, void *state, void *d_ws, size_t ws_bytes, void *stream);
+size_t dq_dense_ws_bytes(int k, uint32_t n_blocks, uint32_t n_ops, uint32_t n_mats);
+
 /* Read-out of `count` states of n qubits each (device pointers in the host
  * array states[]): for every state, P(qubit q = 0) for q < n and the total
  * probability (n + 1 doubles at d_p0 + i * (n + 1); skipped when d_p0 is

@@ -120,6 +120,25 @@ int dgq_tune_lookup(int prec, int n, int out[4]);
  * Returns the number of entries, -1 if the file cannot be read. */
 int dgq_tune_reload(const char *path);
 
+/* Precision form: from now on, flushes of this FP32 register run as dense
+ * sweeps of width k (5, 6 or 7): the recorded gates are grouped into dense
+ * 2^k x 2^k complex matrices, applied on FP32 MFMA with FP32 accumulation
+ * inside 13-qubit tiles (dq_apply_dense).  k = 0 returns to fused
+ * butterflies.  Gates recorded before the call run first, in the current
+ * form.  -1 for a FP64, paged or too small register (k + 5 <= min(n, 13)). */
+int dgq_set_dense(Qureg q, int k);
+
+/* Of the last flush: out = {sweeps, dense ops, device GEMM flops (8 * 2^k *
+ * 2^n per op), host ms of the flush}. */
+int dgq_last_dense_stats(Qureg q, double out[4]);
+
+/* The dense plan of a recording register with width k, for checking on any
+ * host: *n_ops ops in order, op i acting on qubits op_qubits[k*i .. k*i+k-1]
+ * (matrix-index bit j = qubit op_qubits[k*i+j]) with matrix mats[i]
+ * (2^k x 2^k row-major {re, im} doubles).  Returns 0, 1 if cap_ops is too
+ * small (only counts written), -1 on bad arguments. */
+int dgq_recorder_plan_dense(Qureg q, int k, int cap_ops, int *n_ops, int *n_blocks, int *op_qubits, double *mats);
+
 #ifdef __cplusplus
 }
 #endif
