@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--serve-nmax", type=int, default=30)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-dump", type=str, default="", help="write per-request results (JSON lines) here")
+    ap.add_argument("--precision-qubits", type=int, default=32,
+                    help="grid-circuit size of the FP32 MFMA precision-form run (0 = skip)")
+    ap.add_argument("--precision-k", type=str, default="5,6,7")
     return ap.parse_args()
 
 
@@ -230,6 +233,69 @@ def serving(q64, args):
     }
 
 
+def precision_form(q64, args):
+    """Config 4: a 4 x (n/4) grid circuit of depth 20 (sqrt X/Y/W per qubit per
+    cycle, CZ on four edge patterns in rotation), seed 4.  FP64 butterflies on
+    the GPU give the reference state; then an FP32 state runs the same circuit
+    as dense 2^k x 2^k blocks on FP32 MFMA for each k.  Reported per k: time of
+    the dense sweeps (HIP events on the register's stream), dense ops,
+    GEMM flops (8 * 2^k * 2^n per op), MFMA utilisation against the 157.3
+    TFLOP/s FP32 dense peak, HBM GB/s, fidelity |<psi64|psi32>|^2 (normalised)
+    and max |delta amplitude|."""
+    import ctypes as C
+
+    from dgquest_amd import circuits
+    from dgquest_amd import quest as Q
+    from dgquest_amd.engine import Engine
+
+    n = args.precision_qubits
+    cols = n // 4
+    circ = circuits.grid_circuit(4, cols, 20, 4)
+    eng = Engine()
+    q32 = Q.load(1)
+    env = q64.createQuESTEnv()
+    env32 = q32.createQuESTEnv()
+    ref = q64.createQureg(n, env)
+    e0, e1 = eng.event(), eng.event()
+    s64 = q64.dgq_stream(ref)
+    q64.apply_circuit(ref, circ)
+    eng.record(e0, s64)
+    q64.dgq_flush(ref)
+    eng.record(e1, s64)
+    ms64 = eng.elapsed_ms(e0, e1)
+    sweeps64 = q64.plan_stats(ref)["sweeps"]
+    p64 = C.c_int()
+    d64 = q64.dgq_state_pointer(ref, C.byref(p64))
+    rows = {}
+    for k in [int(x) for x in args.precision_k.split(",")]:
+        st = q32.createQureg(n, env32)
+        if q32.dgq_set_dense(st, k) != 0:
+            continue
+        s32 = q32.dgq_stream(st)
+        q32.apply_circuit(st, circ)
+        eng.record(e0, s32)
+        q32.dgq_flush(st)
+        eng.record(e1, s32)
+        ms = eng.elapsed_ms(e0, e1)
+        ds = q32.dense_stats(st)
+        p32 = C.c_int()
+        d32 = q32.dgq_state_pointer(st, C.byref(p32))
+        cmp = eng.compare_states(d64, p64.value, d32, p32.value, n)
+        fid = abs(cmp["inner"]) ** 2 / (cmp["norm_a"] * cmp["norm_b"])
+        hbm = 2 * 8 * (1 << n) * ds["sweeps"] / (ms * 1e-3) / 1e9
+        rows[f"k{k}"] = {"ms": round(ms, 2), "sweeps": ds["sweeps"], "dense_ops": ds["ops"],
+                         "gemm_TFLOP": round(ds["flops"] / 1e12, 2),
+                         "TFLOPs": round(ds["flops"] / (ms * 1e-3) / 1e12, 2),
+                         "mfma_utilisation": round(ds["flops"] / (ms * 1e-3) / 157.3e12, 4),
+                         "hbm_GBps": round(hbm, 1), "fidelity": fid, "raw_overlap_sq": abs(cmp["inner"]) ** 2,
+                         "max_abs_err": cmp["max_abs_diff"], "host_plan_ms": round(ds["flush_ms"], 1)}
+        q32.destroyQureg(st, env32)
+    q64.destroyQureg(ref, env)
+    return {"workload": f"{n}-qubit 4x{cols} grid circuit, depth 20, sqrt X/Y/W + CZ patterns, seed 4",
+            "fp64_reference_ms": round(ms64, 2), "fp64_sweeps": sweeps64, "fp32_dense": rows,
+            "mfma_peak_TFLOPs": 157.3}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -351,6 +417,8 @@ def main():
     }
     if args.paged_bv:
         out["paged_bv"] = paged_bv(q64, env, args.paged_bv - 1, int(args.paged_budget_gib * (1 << 30)))
+    if args.precision_qubits:
+        out["precision_form"] = precision_form(q64, args)
     if args.serve:
         out["serving"] = serving(q64, args)
     if not args.no_cpu_baseline:

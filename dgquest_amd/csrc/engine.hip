@@ -968,14 +968,16 @@ __global__ void write_kernel(typename V2<T>::type *__restrict__ st, uint64_t sta
     st[start + k] = v;
 }
 
+// grid-stride: a launch may not exceed 2^32 threads (n >= 32 states)
 template <typename T>
 __global__ void fill_kernel(typename V2<T>::type *__restrict__ st, uint64_t N, int kind, uint64_t index, double value) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    typename V2<T>::type v;
-    v.x = (kind == DQ_INIT_CLASSICAL) ? (i == index ? T(1) : T(0)) : (T)value;
-    v.y = T(0);
-    __builtin_nontemporal_store(v, st + i);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        typename V2<T>::type v;
+        v.x = (kind == DQ_INIT_CLASSICAL) ? (i == index ? T(1) : T(0)) : (T)value;
+        v.y = T(0);
+        __builtin_nontemporal_store(v, st + i);
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -1399,7 +1401,8 @@ extern "C" int dq_init_state(void *state, int n, int prec, int kind, uint64_t in
     if (!state || n < 1 || n > 40 || (prec != DQ_F32 && prec != DQ_F64)) return fail(DQ_ERR_ARG, "dq_init_state: bad arguments");
     if (kind != DQ_INIT_CLASSICAL && kind != DQ_INIT_PLUS) return fail(DQ_ERR_ARG, "dq_init_state: bad kind %d", kind);
     const uint64_t N = 1ull << n;
-    const unsigned blocks = (unsigned)((N + 255) / 256);
+    const uint64_t want = (N + 255) / 256;
+    const unsigned blocks = (unsigned)(want < (1ull << 20) ? want : (1ull << 20));
     if (prec == DQ_F64)
         hipLaunchKernelGGL(fill_kernel<double>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
                            (V2<double>::type *)state, N, kind, index, value);
@@ -1566,7 +1569,8 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             const size_t lds = multi ? ((size_t)1 << b0.tile_bits) * (split ? elem_bytes(prec) / 2 : elem_bytes(prec)) : 0;
             if (lds > 131072) return fail(DQ_ERR_UNSUPPORTED, "tile of %d bits needs %zu bytes of LDS", b0.tile_bits, lds);
             const uint64_t tiles = 1ull << (n - b0.tile_bits);
-            if (tiles > 0x7fffffffull || nj > 65535) return fail(DQ_ERR_UNSUPPORTED, "grid too large");
+            if (tiles > 0x7fffffffull || nj > 65535 || ((tiles << (6 + b0.wave_bits)) >> 32))
+                return fail(DQ_ERR_UNSUPPORTED, "grid too large (at most 2^32 threads per launch)");
             // (R = 5 keeps the one-pass form: two 32-amplitude tiles do not fit the registers)
             sweep_fn pf = (nj == 1 && b0.reg_bits <= 4 && use_pipe()) ? lookup_pipe(prec, b0.reg_bits, b0.wave_bits, split)
                                                                      : nullptr;

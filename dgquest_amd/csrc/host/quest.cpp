@@ -78,6 +78,9 @@ const char *error_message(ErrorCode c) {
         case E_COLLAPSE_STATE_ZERO_PROB: return "Can't collapse to state with zero probability.";
         case E_INVALID_QUBIT_OUTCOME: return "Invalid measurement outcome -- must be either 0 or 1.";
         case E_DEFINED_ONLY_FOR_STATEVECS: return "Operation valid only for state-vectors.";
+        case E_SECOND_ARG_MUST_BE_STATEVEC: return "Second argument must be a state-vector.";
+        case E_MISMATCHING_QUREG_DIMENSIONS: return "Dimensions of the qubit registers don't match.";
+        case E_MISMATCHING_QUREG_TYPES: return "Registers must both be state-vectors or both be density matrices.";
         default: return "Unknown error.";
     }
 }
@@ -684,6 +687,44 @@ void swapGate(Qureg q, int a, int b) {
     s->p2l[pb] = a;
 }
 
+// ---- overlaps of two registers ---------------------------------------------
+// <a|b>, <a|a>, <b|b>, max |a_i - b_i| of two resident registers, both moved
+// to logical qubit order first, on a's stream.
+static void compare_regs(QuregState *a, QuregState *b, double out[5]) {
+    if (a->paged || b->paged) engine_fatal("overlaps of paged registers", DQ_ERR_UNSUPPORTED);
+    materialize_mapping(a);
+    materialize_mapping(b);
+    ENG(dq_stream_sync(b->stream));
+    const size_t cw = (dq_compare_ws_bytes(a->n) + 255) / 256 * 256;
+    ensure_ws(a, cw + 256);
+    double *d_out = (double *)((unsigned char *)a->d_ws + cw);
+    ENG(dq_compare_states(a->d_state, a->prec, b->d_state, b->prec, a->n, d_out, a->d_ws, cw, a->stream));
+    ENG(dq_memcpy_async(out, d_out, 5 * sizeof(double), DQ_D2H, a->stream));
+    ENG(dq_stream_sync(a->stream));
+}
+
+Complex calcInnerProduct(Qureg bra, Qureg ket) {
+    v_statevec(bra, __func__);
+    v_statevec(ket, __func__);
+    quest_assert(bra.numQubitsRepresented == ket.numQubitsRepresented, E_MISMATCHING_QUREG_DIMENSIONS, __func__);
+    double o[5];
+    compare_regs(st(bra), st(ket), o);
+    Complex c;
+    c.real = (qreal)o[0];
+    c.imag = (qreal)o[1];
+    return c;
+}
+
+qreal calcFidelity(Qureg qureg, Qureg pureState) {
+    quest_assert(qureg.numQubitsRepresented == pureState.numQubitsRepresented, E_MISMATCHING_QUREG_DIMENSIONS,
+                 __func__);
+    quest_assert(!pureState.isDensityMatrix, E_SECOND_ARG_MUST_BE_STATEVEC, __func__);
+    v_statevec(qureg, __func__);
+    double o[5];
+    compare_regs(st(pureState), st(qureg), o);
+    return (qreal)(o[0] * o[0] + o[1] * o[1]);
+}
+
 // ---- results ---------------------------------------------------------------
 Complex getAmp(Qureg qureg, long long int index) {
     v_statevec(qureg, __func__);
@@ -1018,6 +1059,15 @@ int dgq_recorder_plan_dense(Qureg q, int k, int cap_ops, int *n_ops, int *n_bloc
     if (op_qubits) memcpy(op_qubits, D.op_qubits.data(), D.op_qubits.size() * sizeof(int));
     if (mats) memcpy(mats, D.mats.data(), D.mats.size() * sizeof(double));
     return 0;
+}
+
+void *dgq_state_pointer(Qureg q, int *prec) {
+    QuregState *s = q.state;
+    if (!s || s->recorder || s->paged) return nullptr;
+    materialize_mapping(s);
+    ENG(dq_stream_sync(s->stream));
+    if (prec) *prec = s->prec;
+    return s->d_state;
 }
 
 int dgq_paged_info(Qureg q, int *piece_qubits, int *pieces_log2, double stats[10]) {

@@ -132,6 +132,8 @@ class QuEST:
             "collapseToOutcome": (qr, [Qureg, C.c_int, C.c_int]),
             "measure": (C.c_int, [Qureg, C.c_int]),
             "measureWithStats": (C.c_int, [Qureg, C.c_int, C.POINTER(qr)]),
+            "calcInnerProduct": (self.Complex, [Qureg, Qureg]),
+            "calcFidelity": (qr, [Qureg, Qureg]),
         }
         for fname, (res, args) in sig.items():
             f = getattr(L, fname)
@@ -172,6 +174,8 @@ class QuEST:
                                         C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
 
+        L.dgq_state_pointer.restype = C.c_void_p
+        L.dgq_state_pointer.argtypes = [Qureg, C.POINTER(C.c_int)]
         L.dgq_set_dense.restype = C.c_int
         L.dgq_set_dense.argtypes = [Qureg, C.c_int]
         L.dgq_last_dense_stats.restype = C.c_int

@@ -125,6 +125,8 @@ qreal getImagAmp(Qureg qureg, long long int index);
 qreal getProbAmp(Qureg qureg, long long int index);
 qreal calcProbOfOutcome(Qureg qureg, const int measureQubit, int outcome);
 qreal calcTotalProb(Qureg qureg);
+Complex calcInnerProduct(Qureg bra, Qureg ket);       /* <bra|ket> */
+qreal calcFidelity(Qureg qureg, Qureg pureState);     /* |<pureState|qureg>|^2 */
 
 /* ---- measurement (seeded by seedQuEST / seedQuESTDefault) -------------- */
 qreal collapseToOutcome(Qureg qureg, const int measureQubit, int outcome);

@@ -139,6 +139,13 @@ int dgq_last_dense_stats(Qureg q, double out[4]);
  * small (only counts written), -1 on bad arguments. */
 int dgq_recorder_plan_dense(Qureg q, int k, int cap_ops, int *n_ops, int *n_blocks, int *op_qubits, double *mats);
 
+/* Flush q, move its qubits to their logical positions, wait, and return the
+ * device address of its state (2^n interleaved complex amplitudes of the
+ * precision written to *prec, DQ_F32 / DQ_F64), for engine calls such as
+ * dq_compare_states across registers of the two precisions.  NULL for a
+ * recording or paged register.  Valid until the next call on q. */
+void *dgq_state_pointer(Qureg q, int *prec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,3 +83,48 @@ def test_dense_refuses_fp64_and_small(quest, env, quest32, env32):
     assert quest32.dgq_set_dense(q, 6) == -1
     assert quest32.dgq_set_dense(q, 5) == 0
     quest32.destroyQureg(q, env32)
+
+
+def test_inner_product_and_fidelity(quest, env, oracle):
+    """calcInnerProduct / calcFidelity (declared by the reference, defined
+    here) against numpy on the oracle's states, with relabelled qubits."""
+    n = 14
+    rng = np.random.default_rng(3)
+    c1 = H.random_circuit(rng, n, 80)
+    c2 = H.random_circuit(rng, n, 80)
+    a = quest.createQureg(n, env)
+    b = quest.createQureg(n, env)
+    quest.apply_circuit(a, c1)
+    quest.apply_circuit(b, c2)
+    ra, rb = oracle_state(oracle, n, c1), oracle_state(oracle, n, c2)
+    ip = quest.calcInnerProduct(a, b)
+    want = np.vdot(ra, rb)
+    assert abs(complex(ip.real, ip.imag) - want) <= 1e-12
+    assert abs(quest.calcFidelity(a, b) - abs(np.vdot(rb, ra)) ** 2) <= 1e-12
+    assert abs(quest.calcFidelity(a, a) - 1.0) <= 1e-12
+    quest.destroyQureg(a, env)
+    quest.destroyQureg(b, env)
+
+
+def test_cross_precision_compare(quest, env, quest32, env32):
+    """The same circuit in FP64 and in FP32 dense mode, compared on the device."""
+    from dgquest_amd.engine import Engine
+    import ctypes as C
+
+    n = 16
+    circ = CC.grid_circuit(4, 4, 10, 7)
+    a = quest.createQureg(n, env)
+    b = quest32.createQureg(n, env32)
+    assert quest32.dgq_set_dense(b, 6) == 0
+    quest.apply_circuit(a, circ)
+    quest32.apply_circuit(b, circ)
+    pa, pb = C.c_int(), C.c_int()
+    da = quest.dgq_state_pointer(a, C.byref(pa))
+    db = quest32.dgq_state_pointer(b, C.byref(pb))
+    r = Engine().compare_states(da, pa.value, db, pb.value, n)
+    sa, sb = quest.state_vector(a), quest32.state_vector(b)
+    assert abs(r["inner"] - np.vdot(sa, sb)) <= 1e-9
+    assert abs(r["max_abs_diff"] - np.abs(sa - sb).max()) <= 1e-12
+    assert abs(r["inner"]) ** 2 >= 1 - 2e-6
+    quest.destroyQureg(a, env)
+    quest32.destroyQureg(b, env32)
