@@ -797,7 +797,8 @@ compare_stage1(const typename V2<TA>::type *__restrict__ a, const typename V2<TB
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 #define DENSE_T_MAX 13
-#define DENSE_THREADS 256
+#define DENSE_THREADS 512
+#define DENSE_WAVES (DENSE_THREADS / 64)
 
 // LDS slot of tile element e: the low 5 bits are xored with bits 5..9 and
 // 10..14, spreading power-of-two strides over the 32 banks (a bijection:
@@ -810,74 +811,81 @@ __device__ __forceinline__ uint32_t deposit_bits(uint32_t v, const uint32_t *pos
     return r;
 }
 
+// One dense op on the tile in LDS.  Work item = (32 columns, RB row blocks of
+// 32); one item per wave at T = 13.  The row offsets of the B operand step
+// through the op's bits 1..k-1 with the submask increment
+// x' = ((x | ~S) + 1) & S (op bits ascending), so no table is read per MFMA.
 template <int KL>
 __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restrict__ im, uint32_t T,
                                          const dq_dense_op &op, const float *__restrict__ mat,
-                                         uint32_t *__restrict__ rowoff, uint32_t *__restrict__ coloff, uint32_t tid) {
+                                         uint32_t *__restrict__ coloff, uint32_t tid) {
     constexpr int K = 1 << KL, M = 2 * K, MB = M / 32;
-    constexpr int RB = MB < 4 ? MB : 4;          // row blocks per work item
-    constexpr int IMAX = KL == 5 ? 2 : 1;        // work items per wave at T = 13
+    constexpr int RB = MB < 2 ? MB : 2;          // row blocks per work item
     const uint32_t NC = 1u << (T - KL);
     const uint32_t NCH = NC / 32;
     const uint32_t items = NCH * (MB / RB);
     const uint32_t w = tid >> 6, lane = tid & 63;
-    // index tables of this op: matrix index -> tile offset, column -> tile offset
-    uint32_t obits[8], cbits[DENSE_T_MAX];
+    uint32_t cbits[DENSE_T_MAX];
     uint32_t used = 0;
-    for (int j = 0; j < KL; j++) { obits[j] = op.bits[j]; used |= 1u << op.bits[j]; }
+    for (int j = 0; j < KL; j++) used |= 1u << op.bits[j];
     int nc = 0;
     for (uint32_t b = 0; b < T; b++)
         if (!((used >> b) & 1u)) cbits[nc++] = b;
+    const uint32_t b0 = 1u << op.bits[0];
+    const uint32_t S = used & ~b0;  // tile bits of matrix-index bits 1..k-1
     __syncthreads();  // the previous op's writes and table reads are done
-    for (uint32_t j = tid; j < (uint32_t)K; j += DENSE_THREADS) rowoff[j] = deposit_bits(j, obits, KL);
     for (uint32_t c = tid; c < NC; c += DENSE_THREADS) coloff[c] = deposit_bits(c, cbits, nc);
     __syncthreads();
-    v16f acc[IMAX][RB];
+    v16f acc[RB];
 #pragma unroll
-    for (int i = 0; i < IMAX; i++)
-#pragma unroll
-        for (int r = 0; r < RB; r++)
-            for (int v = 0; v < 16; v++) acc[i][r][v] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < IMAX; i++) {
-        const uint32_t item = w + 4u * i;
-        if (item >= items) break;  // uniform over the wave
-        const uint32_t ch = item % NCH, rg = item / NCH;
-        const uint32_t col = ch * 32 + (lane & 31u);
-        const uint32_t cofs = coloff[col];
+    for (int r = 0; r < RB; r++)
+        for (int v = 0; v < 16; v++) acc[r][v] = 0.0f;
+    const uint32_t item = w;
+    const bool busy = item < items;  // uniform over the wave
+    uint32_t cofs = 0;
+    uint32_t rg = 0;
+    if (busy) {
+        const uint32_t ch = item % NCH;
+        rg = item / NCH;
+        cofs = coloff[ch * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u);
         const float *__restrict__ ma = mat + (size_t)(rg * RB) * (M / 2) * 64 + lane;
+        // rows 2p + (lane / 32): p < K/2 real parts, p >= K/2 imaginary parts
+#pragma unroll 2
+        for (int half = 0; half < 2; half++) {
+            const float *__restrict__ src = half ? im : re;
+            uint32_t x = 0;
 #pragma unroll 4
-        for (int p = 0; p < M / 2; p++) {
-            const uint32_t row = 2 * p + (lane >> 5);
-            const uint32_t e = dswz(rowoff[row & (K - 1)] | cofs);
-            const float b = (row >> KL) ? im[e] : re[e];
+            for (int pp = 0; pp < K / 2; pp++) {
+                const int p = half * (K / 2) + pp;
+                const float bv = src[dswz(x | cofs)];
+                x = ((x | ~S) + 1u) & S;
 #pragma unroll
-            for (int r = 0; r < RB; r++) {
-                const float a = ma[((size_t)r * (M / 2) + p) * 64];
-                acc[i][r] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[i][r], 0, 0, 0);
+                for (int r = 0; r < RB; r++) {
+                    const float a = ma[((size_t)r * (M / 2) + p) * 64];
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc[r], 0, 0, 0);
+                }
             }
         }
     }
     __syncthreads();  // every wave has read the whole tile: write in place
-#pragma unroll
-    for (int i = 0; i < IMAX; i++) {
-        const uint32_t item = w + 4u * i;
-        if (item >= items) break;
-        const uint32_t ch = item % NCH, rg = item / NCH;
-        const uint32_t cofs = coloff[ch * 32 + (lane & 31u)];
+    if (busy) {
+        const uint32_t ch = item % NCH;
+        const uint32_t cofs0 = coloff[ch * 32 + (lane & 31u)];
+        uint32_t obits[8];
+        for (int j = 0; j < KL; j++) obits[j] = op.bits[j];
 #pragma unroll
         for (int r = 0; r < RB; r++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
                 const uint32_t row = (rg * RB + r) * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const uint32_t e = dswz(rowoff[row & (K - 1)] | cofs);
-                if (row >> KL) im[e] = acc[i][r][v];
-                else re[e] = acc[i][r][v];
+                const uint32_t e = dswz(deposit_bits(row & (K - 1), obits, KL) | cofs0);
+                if (row >> KL) im[e] = acc[r][v];
+                else re[e] = acc[r][v];
             }
     }
 }
 
-// One dense block over every tile of an FP32 state: grid = tiles, 256 threads.
+// One dense block over every tile of an FP32 state: grid = tiles, 512 threads.
 template <int KL>
 __global__ void __launch_bounds__(DENSE_THREADS)
 dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restrict__ ops, const float *__restrict__ mats,
@@ -887,8 +895,7 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     const uint32_t NT = 1u << T;
     float *re = reinterpret_cast<float *>(lds_raw);
     float *im = re + NT;
-    uint32_t *rowoff = reinterpret_cast<uint32_t *>(im + NT);
-    uint32_t *coloff = rowoff + 128;
+    uint32_t *coloff = reinterpret_cast<uint32_t *>(im + NT);
     const uint32_t tid = threadIdx.x;
     const uint64_t base = tile_base(blk->nontile_mask, blockIdx.x);
     uint64_t hq[DENSE_T_MAX];
@@ -906,7 +913,7 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     const size_t msz = (size_t)(2u << KL) * (2u << KL);
     for (uint32_t o = 0; o < blk->op_count; o++) {
         const dq_dense_op op = ops[blk->op_begin + o];
-        dense_op<KL>(re, im, T, op, mats + op.mat * msz, rowoff, coloff, tid);
+        dense_op<KL>(re, im, T, op, mats + op.mat * msz, coloff, tid);
     }
     __syncthreads();
     for (uint32_t e = tid; e < NT; e += DENSE_THREADS) {
@@ -1766,7 +1773,7 @@ extern "C" int dq_compare_states(const void *a, int prec_a, const void *b, int p
 // ---- dense sweeps -------------------------------------------------------------
 namespace {
 size_t dense_mat_floats(int k) { return (size_t)(2u << k) * (2u << k); }
-size_t dense_lds_bytes(uint32_t T) { return 2 * ((size_t)4 << T) + (128 + 256) * sizeof(uint32_t); }
+size_t dense_lds_bytes(uint32_t T) { return 2 * ((size_t)4 << T) + 256 * sizeof(uint32_t); }
 }  // namespace
 
 extern "C" size_t dq_dense_ws_bytes(int k, uint32_t n_blocks, uint32_t n_ops, uint32_t n_mats) {
@@ -1798,11 +1805,9 @@ extern "C" int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_bl
             return fail(DQ_ERR_ARG, "dense block %u: tile and non-tile masks inconsistent", i);
         if ((uint64_t)b.op_begin + b.op_count > n_ops) return fail(DQ_ERR_ARG, "dense block %u: op range", i);
         for (uint32_t o = b.op_begin; o < b.op_begin + b.op_count; o++) {
-            uint32_t u = 0;
-            for (int j = 0; j < k; j++) {
-                if (ops[o].bits[j] >= T || ((u >> ops[o].bits[j]) & 1)) return fail(DQ_ERR_ARG, "dense op %u: bad bits", o);
-                u |= 1u << ops[o].bits[j];
-            }
+            for (int j = 0; j < k; j++)
+                if (ops[o].bits[j] >= T || (j && ops[o].bits[j] <= ops[o].bits[j - 1]))
+                    return fail(DQ_ERR_ARG, "dense op %u: bits must be ascending tile bits", o);
             if (ops[o].mat >= n_mats) return fail(DQ_ERR_ARG, "dense op %u: matrix index", o);
         }
         if (n - T > 31) return fail(DQ_ERR_UNSUPPORTED, "dense block %u: grid too large", i);

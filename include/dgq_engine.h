@@ -223,7 +223,7 @@ typedef struct dq_dense_block {
 } dq_dense_block;
 
 typedef struct dq_dense_op {
-    uint32_t bits[8];     /* tile bits of matrix-index bits 0..k-1 */
+    uint32_t bits[8];     /* tile bits of matrix-index bits 0..k-1, ascending */
     uint64_t mat;         /* index of the op's matrix in the call's matrix array */
 } dq_dense_op;
 
