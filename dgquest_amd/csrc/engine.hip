@@ -15,6 +15,7 @@
 // Layout: 2^n interleaved complex amplitudes; qubit q = bit q of the index.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -1229,7 +1230,9 @@ extern "C" size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32
 // Device program of one phase (format and op codes in gen_ops_asm.py /
 // ops_asm.inc): register conditions resolved into op variants or masks,
 // consecutive unconditioned shears on distinct register bits merged into one
-// op, a WG op in front of each op with a workgroup condition, END last.
+// op, unconditioned diagonal ops folded into the table of their run and that
+// table fused with the shears after it, a WG op in front of each op with a
+// workgroup condition, END last.
 namespace {
 struct ProgBuilder {
     int prec, R, NR, ncodes;
@@ -1292,12 +1295,125 @@ struct ProgBuilder {
         }
         return code(gen);
     }
+    // Value i of op o's coefficient array, in double.
+    double val(const dq_op &o, int i) const {
+        if (prec == DQ_F64) {
+            double v;
+            memcpy(&v, reinterpret_cast<const unsigned char *>(o.m) + 8 * i, 8);
+            return v;
+        }
+        float v;
+        memcpy(&v, reinterpret_cast<const unsigned char *>(o.m) + 4 * i, 4);
+        return (double)v;
+    }
+    void put(std::vector<unsigned char> &v, double x) const {
+        const size_t at = v.size();
+        v.resize(at + ts);
+        if (prec == DQ_F64) memcpy(v.data() + at, &x, 8);
+        else {
+            const float f = (float)x;
+            memcpy(v.data() + at, &f, 4);
+        }
+    }
+    // A table op (or, with shear slots, a table+shears op) for the diagonal
+    // (tr, ti) over the register index: HALF when every factor that is not 1
+    // has one register bit set, FULL when at least 3/4 of them are not 1, GEN
+    // (a mask of the factors to apply) otherwise.  False if every factor is 1.
+    bool encode_table(std::vector<unsigned char> &op, const std::vector<double> &tr, const std::vector<double> &ti,
+                      const std::vector<unsigned char> *slots, uint32_t shear_mask) {
+        std::vector<unsigned char> table;
+        for (int e = 0; e < NR; e++) {
+            put(table, tr[e]);
+            put(table, ti[e]);
+        }
+        uint32_t ne1 = 0;  // factors that are not exactly 1 in the state precision
+        for (int e = 0; e < NR; e++) {
+            double r, i;
+            if (prec == DQ_F64) {
+                memcpy(&r, table.data() + e * 16, 8);
+                memcpy(&i, table.data() + e * 16 + 8, 8);
+            } else {
+                float fr, fi;
+                memcpy(&fr, table.data() + e * 8, 4);
+                memcpy(&fi, table.data() + e * 8 + 4, 4);
+                r = fr;
+                i = fi;
+            }
+            if (!(r == 1.0 && i == 0.0)) ne1 |= 1u << e;
+        }
+        if (!ne1) return false;
+        int half = -1;
+        for (int r = 0; r < R && half < 0; r++)
+            if ((ne1 & ~jmask(1u << r, 1u << r, -1)) == 0) half = r;
+        std::vector<unsigned char> body;
+        int c;
+        uint32_t gen = 0;
+        if (half >= 0) {
+            for (int e = 0; e < NR; e++)
+                if ((e >> half) & 1)
+                    body.insert(body.end(), table.begin() + (size_t)e * 2 * ts, table.begin() + (size_t)(e + 1) * 2 * ts);
+            c = slots ? code("TSH_HALF", half) : code("TABLE_HALF", half);
+        } else if (4 * __builtin_popcount(ne1) >= 3 * NR) {
+            body = table;
+            c = slots ? code("TSH_FULL") : code("TABLE_FULL");
+        } else {
+            body = table;
+            gen = ne1;
+            c = slots ? code("TSH_GEN") : code("TABLE_GEN");
+        }
+        if (slots) {
+            body.insert(body.end(), slots->begin(), slots->end());
+            encode(op, c, shear_mask, gen, 0, body.data(), body.size());
+        } else {
+            encode(op, c, gen, 0, 0, body.data(), body.size());
+        }
+        return true;
+    }
+    // Diagonal ops commute, so every diagonal op without a lane or workgroup
+    // condition in a run of diagonal ops that holds a table is multiplied into
+    // that run's table; the table is emitted before the next non-diagonal op,
+    // fused with it when that op is a run of unconditioned shears.
     void build(const dq_op *ops, uint32_t count, uint32_t &n_out) {
         n_out = 0;
         std::vector<unsigned char> op;
+        std::vector<double> tr(NR, 1.0), ti(NR, 0.0);
+        bool pending = false;  // tr/ti hold a table not yet emitted
+        auto is_diag = [](const dq_op &o) {
+            return o.kind == DQ_OP_NEG || o.kind == DQ_OP_PHASE || o.kind == DQ_OP_DIAGR;
+        };
+        auto mul = [&](uint32_t c, uint32_t v, double fr, double fi) {
+            for (int j = 0; j < NR; j++)
+                if (((uint32_t)j & c) == v) {
+                    const double r = tr[j] * fr - ti[j] * fi, i = tr[j] * fi + ti[j] * fr;
+                    tr[j] = r;
+                    ti[j] = i;
+                }
+        };
+        auto flush = [&](const std::vector<unsigned char> *slots, uint32_t shear_mask) {
+            op.clear();
+            const bool any = pending && encode_table(op, tr, ti, slots, shear_mask);
+            std::fill(tr.begin(), tr.end(), 1.0);
+            std::fill(ti.begin(), ti.end(), 0.0);
+            pending = false;
+            if (!any) {
+                if (!slots) return;
+                op.clear();
+                encode(op, code("SHEARS"), shear_mask, 0, 0, slots->data(), slots->size());
+            }
+            emit(op, 0, 0);
+            n_out++;
+        };
+        bool fold_run = false;  // the current run of diagonal ops holds an unconditioned table
+        bool in_run = false;
         for (uint32_t k = 0; k < count; k++) {
             const dq_op &o = ops[k];
             const unsigned char *m = reinterpret_cast<const unsigned char *>(o.m);
+            if (is_diag(o) && !in_run) {
+                fold_run = false;
+                for (uint32_t k2 = k; k2 < count && is_diag(ops[k2]); k2 += ops[k2].kind == DQ_OP_DIAGR ? 1 + ops[k2].tbit : 1)
+                    if (ops[k2].kind == DQ_OP_DIAGR && !ops[k2].c_wg) fold_run = true;
+            }
+            in_run = is_diag(o);
             op.clear();
             switch (o.kind) {
                 case DQ_OP_SHEAR: {
@@ -1311,71 +1427,72 @@ struct ProgBuilder {
                         k2++;
                     }
                     if (k2 == k) {  // a shear with a workgroup condition: on its own
+                        flush(nullptr, 0);
+                        op.clear();
                         mask = 1u << o.tbit;
                         memcpy(slots.data() + 2 * o.tbit * ts, o.m, 2 * ts);
-                        k2 = k + 1;
+                        encode(op, code("SHEARS"), mask, 0, 0, slots.data(), slots.size());
+                        emit(op, o.c_wg, o.v_wg);
+                        n_out++;
+                        continue;
                     }
-                    encode(op, code("SHEARS"), mask, 0, 0, slots.data(), slots.size());
-                    emit(op, k2 == k + 1 ? o.c_wg : 0, k2 == k + 1 ? o.v_wg : 0);
-                    n_out++;
+                    flush(&slots, mask);
                     k = k2 - 1;
                     continue;
                 }
                 case DQ_OP_DIAGR: {
-                    std::vector<unsigned char> table((size_t)NR * 2 * ts);
-                    uint32_t ne1 = 0;  // entries that are not exactly 1
-                    for (int e = 0; e < NR; e++) {
-                        const unsigned char *ent = reinterpret_cast<const unsigned char *>(ops[k + 1 + e / 4].m) + (e % 4) * 2 * ts;
-                        memcpy(table.data() + (size_t)e * 2 * ts, ent, 2 * ts);
-                        bool one;
-                        if (prec == DQ_F64) {
-                            double v[2];
-                            memcpy(v, ent, sizeof v);
-                            one = v[0] == 1.0 && v[1] == 0.0;
-                        } else {
-                            float v[2];
-                            memcpy(v, ent, sizeof v);
-                            one = v[0] == 1.0f && v[1] == 0.0f;
+                    if (!o.c_wg) {
+                        for (int e = 0; e < NR; e++) {
+                            const dq_op &d = ops[k + 1 + e / 4];
+                            mul((uint32_t)(NR - 1), (uint32_t)e, val(d, 2 * (e % 4)), val(d, 2 * (e % 4) + 1));
                         }
-                        if (!one) ne1 |= 1u << e;
-                    }
-                    int half = -1;
-                    for (int r = 0; r < R && half < 0; r++)
-                        if (ne1 && (ne1 & ~jmask(1u << r, 1u << r, -1)) == 0) half = r;
-                    if (half >= 0) {
-                        std::vector<unsigned char> compact;
-                        for (int e = 0; e < NR; e++)
-                            if ((e >> half) & 1)
-                                compact.insert(compact.end(), table.begin() + (size_t)e * 2 * ts,
-                                               table.begin() + (size_t)(e + 1) * 2 * ts);
-                        encode(op, code("TABLE_HALF", half), 0, 0, 0, compact.data(), compact.size());
-                    } else if (4 * __builtin_popcount(ne1) >= 3 * NR) {
-                        encode(op, code("TABLE_FULL"), 0, 0, 0, table.data(), table.size());
-                    } else {
-                        encode(op, code("TABLE_GEN"), ne1, 0, 0, table.data(), table.size());
-                    }
-                    if (ne1) {
-                        emit(op, o.c_wg, o.v_wg);
-                        n_out++;
+                        pending = true;
+                    } else {  // a table under a workgroup condition: its own op (it commutes with the pending one)
+                        std::vector<double> wr(NR), wi(NR);
+                        for (int e = 0; e < NR; e++) {
+                            const dq_op &d = ops[k + 1 + e / 4];
+                            wr[e] = val(d, 2 * (e % 4));
+                            wi[e] = val(d, 2 * (e % 4) + 1);
+                        }
+                        if (encode_table(op, wr, wi, nullptr, 0)) {
+                            emit(op, o.c_wg, o.v_wg);
+                            n_out++;
+                        }
                     }
                     k += o.tbit;
                     continue;
                 }
                 case DQ_OP_NEG:
+                    if (fold_run && !o.c_thr && !o.c_wg) {
+                        mul(o.c_reg, o.v_reg, -1.0, 0.0);
+                        pending = true;
+                        continue;
+                    }
                     encode(op, cond_variant("NEG_ALL", "NEG_B", "NEG_BB", "NEG_GEN", o.c_reg, o.v_reg),
                            jmask(o.c_reg, o.v_reg, -1), o.c_thr, o.v_thr, nullptr, 0);
                     break;
                 case DQ_OP_PHASE:
+                    if (fold_run && !o.c_thr && !o.c_wg) {
+                        mul(o.c_reg, o.v_reg, val(o, 0), val(o, 1));
+                        pending = true;
+                        continue;
+                    }
                     encode(op, cond_variant("PHASE_ALL", "PHASE_B", "PHASE_BB", "PHASE_GEN", o.c_reg, o.v_reg),
                            jmask(o.c_reg, o.v_reg, -1), o.c_thr, o.v_thr, m, 2 * ts);
                     break;
                 case DQ_OP_MAT:
+                    flush(nullptr, 0);
+                    op.clear();
                     encode(op, code("MAT", (int)o.tbit), jmask(o.c_reg, o.v_reg, (int)o.tbit), o.c_thr, o.v_thr, m, 8 * ts);
                     break;
                 case DQ_OP_SWAP:
+                    flush(nullptr, 0);
+                    op.clear();
                     encode(op, code("SWAP", (int)o.tbit), jmask(o.c_reg, o.v_reg, (int)o.tbit), o.c_thr, o.v_thr, nullptr, 0);
                     break;
                 case DQ_OP_RMAT:
+                    flush(nullptr, 0);
+                    op.clear();
                     encode(op, code("RMAT", (int)o.tbit), 0, 0, 0, m, 4 * ts);
                     break;
                 default:
@@ -1384,6 +1501,7 @@ struct ProgBuilder {
             emit(op, o.c_wg, o.v_wg);
             n_out++;
         }
+        flush(nullptr, 0);
         op.clear();
         encode(op, code("END"), 0, 0, 0, nullptr, 0);
         out.insert(out.end(), op.begin(), op.end());

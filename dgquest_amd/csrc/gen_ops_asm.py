@@ -23,19 +23,27 @@ Device program (built by the engine's host code from dq_op arrays): each op is
 a 16-byte header followed by its coefficients in the amplitude type; the
 size of each op code is fixed (op_size below).
     u32 code     index into the jump table
-    u32 mask     SHEARS: register bits sheared; *_GEN / MAT / SWAP: register
-                 indices (pair ops: lower indices) that satisfy the condition
-    u32 c_thr    lane condition mask (0: every lane) ...
+    u32 mask     SHEARS / TSH_*: register bits sheared; *_GEN / MAT / SWAP:
+                 register indices (pair ops: lower indices) that satisfy the
+                 condition
+    u32 c_thr    lane condition mask (0: every lane); TSH_GEN: the table's
+                 register indices ...
     u32 v_thr    ... and value; WG: bytes of the guarded op
     coefficients from byte 16:
       WG        c_wg (u64), v_wg (u64)
       SHEARS    per register bit b: u_b, l_b at 16 + 2 * b * sizeof(T)
       TABLE_*   2^R factors (HALF: the 2^(R-1) with the bit set, in order)
+      TSH_*     a TABLE_* table, then the SHEARS slots: the diagonal, then the
+                shears, in one op
       PHASE_*   re, im          MAT  u00 u01 u10 u11 (re, im)     RMAT m0..m3
 Scalar registers s36..s71 are reserved by the asm (clobbered; the compiler
-keeps the SGPRs above s71 for itself at high occupancy):
-    s[36:51] second load buffer / scratch    s[52:67] header + first coefficients
-    s[68:69] next op                          s[70:71] saved exec
+keeps the SGPRs above s71 for itself, and at 8 waves per SIMD it may not use
+any above s71):
+    s[36:51] second load buffer / scratch (s[36:37]: saved exec)
+    s[52:67] header + first coefficients     s68 byte offset of the current op
+    s69      saved mask                       s[70:71] jump-table address
+Each handler ends by loading the next op's header (program base + s68) and
+jumping through the table, with no branch back to a common loop head.
 Only scalar loads touch memory; every store is a vector register write.
 
 Usage: python3 gen_ops_asm.py > ops_asm.inc   (the Makefile does this)
@@ -43,7 +51,8 @@ Usage: python3 gen_ops_asm.py > ops_asm.inc   (the Makefile does this)
 import itertools
 import sys
 
-BUF2, HDR, NEXT, EXEC = 36, 52, 68, 70
+BUF2, HDR, OFF, SCR, TAB = 36, 52, 68, 69, 70
+EXEC = BUF2             # saved exec of a lane-conditioned op (s[36:37])
 COEF = HDR + 4          # coefficients start right after the 16-byte header
 CLOBBER = range(36, 72)
 RMAX = 5
@@ -65,6 +74,8 @@ def op_codes(R):
     codes += [("MAT", (b,)) for b in range(R)]
     codes += [("SWAP", (b,)) for b in range(R)]
     codes += [("RMAT", (b,)) for b in range(R)]
+    codes += [("TSH_FULL", ()), ("TSH_GEN", ())]
+    codes += [("TSH_HALF", (r,)) for r in range(R)]
     return codes
 
 
@@ -81,6 +92,10 @@ def op_size(name, R, ts):
         b = (2 << R) * ts
     elif name == "TABLE_HALF":
         b = (1 << R) * ts
+    elif name in ("TSH_FULL", "TSH_GEN"):
+        b = (2 << R) * ts + 2 * R * ts
+    elif name == "TSH_HALF":
+        b = (1 << R) * ts + 2 * R * ts
     elif base == "PHASE":
         b = 2 * ts
     elif name == "MAT":
@@ -124,11 +139,39 @@ class Gen:
     def lbl(self, name):
         return f".L{name}_%="
 
+    def dispatch(self):
+        """load the header of the op at s68 and jump: table entry = s72 + 4 * code"""
+        self.e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], %[prog], s{OFF}")
+        self.e("s_waitcnt lgkmcnt(0)")
+        self.e(f"s_lshl2_add_u32 s36, s{HDR}, s{TAB}")
+        self.e(f"s_addc_u32 s37, s{TAB + 1}, 0")
+        self.e("s_setpc_b64 s[36:37]")
+
     def tail(self, name):
         size = op_size(name, self.R, self.ts)
-        self.e(f"s_add_u32 s{NEXT}, s{NEXT}, {size}")
-        self.e(f"s_addc_u32 s{NEXT + 1}, s{NEXT + 1}, 0")
-        self.e(f"s_branch {self.lbl('op')}")
+        self.e(f"s_add_u32 s{OFF}, s{OFF}, {size}")
+        self.dispatch()
+
+    def load(self, dst, off):
+        """64 bytes of the current op from byte `off` into s[dst:dst+15]"""
+        self.e(f"s_load_dwordx16 s[{dst}:{dst + 15}], %[prog], s{OFF} offset:{hex(off)}")
+
+    def load_bytes(self, off, nbytes):
+        """nbytes (a multiple of 4, at most 128) of the current op from byte
+        `off` into BUF2, then HDR; returns the first SGPR of each dword"""
+        regs = list(range(BUF2, BUF2 + 16)) + list(range(HDR, HDR + 16))
+        nd = nbytes // 4
+        i = 0
+        while i < nd:
+            for w in (16, 8, 4, 2, 1):
+                # a load stays inside one buffer and starts at a multiple of min(w, 4)
+                if w <= nd - i and (i % 16) + w <= 16 and regs[i] % min(w, 4) == 0:
+                    break
+            op = "s_load_dword" if w == 1 else f"s_load_dwordx{w}"
+            dst = f"s{regs[i]}" if w == 1 else f"s[{regs[i]}:{regs[i] + w - 1}]"
+            self.e(f"{op} {dst}, %[prog], s{OFF} offset:{hex(off + 4 * i)}")
+            i += w
+        return regs[:nd]
 
     # --- arithmetic building blocks ------------------------------------------
     def pairs(self, B):
@@ -154,7 +197,7 @@ class Gen:
                 self.e(f"v_fma_{f} {self.im(j)}, {pr_of(j)}, {self.im(j)}, %[t{2 * n + 1}]")
 
     def lanes_on(self, tag):
-        """exec &= lanes with (tid & c_thr) == v_thr; exec saved in s[70:71]."""
+        """exec &= lanes with (tid & c_thr) == v_thr; exec saved in s[36:37]."""
         self.e(f"s_mov_b64 {sp(EXEC)}, exec")
         self.e(f"s_cmp_eq_u32 s{HDR + 2}, 0")
         self.e(f"s_cbranch_scc1 {self.lbl(tag)}")
@@ -181,7 +224,7 @@ class Gen:
         bufs = [HDR, BUF2]
         where = {}
         for i, c in enumerate(chunks[:2]):
-            self.e(f"s_load_dwordx16 s[{bufs[i]}:{bufs[i] + 15}], {sp(NEXT)}, {hex(16 + 64 * c)}")
+            self.load(bufs[i], 16 + 64 * c)
             where[c] = bufs[i]
         self.e("s_waitcnt lgkmcnt(0)")
         for i, c in enumerate(chunks):
@@ -192,7 +235,7 @@ class Gen:
                            lambda j, buf=buf, pos=pos: self.sval(buf, 2 * (pos[j] % per) + 1))
             if i + 2 < len(chunks):  # refill this buffer with the chunk after next
                 c2 = chunks[i + 2]
-                self.e(f"s_load_dwordx16 s[{buf}:{buf + 15}], {sp(NEXT)}, {hex(16 + 64 * c2)}")
+                self.load(buf, 16 + 64 * c2)
                 where[c2] = buf
             if i % 2 == 1 and i + 1 < len(chunks):
                 self.e("s_waitcnt lgkmcnt(0)")
@@ -203,26 +246,18 @@ class Gen:
         self.e(f"s_and_b64 s[36:37], %[base], {sp(COEF)}")
         self.e(f"s_cmp_eq_u64 s[36:37], {sp(COEF + 2)}")
         self.e(f"s_cselect_b32 s38, 0, s{HDR + 3}")
-        self.e(f"s_add_u32 s{NEXT}, s{NEXT}, s38")
-        self.e(f"s_addc_u32 s{NEXT + 1}, s{NEXT + 1}, 0")
+        self.e(f"s_add_u32 s{OFF}, s{OFF}, s38")
         self.tail("WG")
 
-    def h_shears(self):
+    def shear_bits(self, mask_reg, coef, tag):
+        """the shears of every register bit set in s<mask_reg>; coef(B) gives
+        the (u, l) operands of bit B"""
         f = self.f
-        if self.d and self.R >= 4:  # slots 3 (and 4) beyond the first load
-            self.e(f"s_load_dwordx8 s[{BUF2}:{BUF2 + 7}], {sp(NEXT)}, 0x40")
-            self.e("s_waitcnt lgkmcnt(0)")
         for B in range(self.R):
-            if self.d:
-                if B < 3:
-                    u, l = sp(COEF + 4 * B), sp(COEF + 4 * B + 2)
-                else:
-                    u, l = sp(BUF2 + 4 * (B - 3)), sp(BUF2 + 4 * (B - 3) + 2)
-            else:
-                u, l = f"s{COEF + 2 * B}", f"s{COEF + 2 * B + 1}"
-            tag = f"sh{B}"
-            self.e(f"s_bitcmp1_b32 s{HDR + 1}, {B}")
-            self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
+            u, l = coef(B)
+            t = f"{tag}{B}"
+            self.e(f"s_bitcmp1_b32 s{mask_reg}, {B}")
+            self.e(f"s_cbranch_scc0 {self.lbl(t)}")
             P = self.pairs(B)
             for j0, j1 in P:
                 self.e(f"v_fmac_{f} {self.re(j0)}, {u}, {self.re(j1)}")
@@ -230,8 +265,50 @@ class Gen:
             for j0, j1 in P:
                 self.e(f"v_fmac_{f} {self.re(j1)}, {l}, {self.re(j0)}")
                 self.e(f"v_fmac_{f} {self.im(j1)}, {l}, {self.im(j0)}")
-            self.e(f"{self.lbl(tag)}:")
+            self.e(f"{self.lbl(t)}:")
+
+    def h_shears(self):
+        if self.d and self.R >= 4:  # slots 3 (and 4) beyond the first load
+            self.e(f"s_load_dwordx8 s[{BUF2}:{BUF2 + 7}], %[prog], s{OFF} offset:0x40")
+            self.e("s_waitcnt lgkmcnt(0)")
+
+        def coef(B):
+            if self.d:
+                if B < 3:
+                    return sp(COEF + 4 * B), sp(COEF + 4 * B + 2)
+                return sp(BUF2 + 4 * (B - 3)), sp(BUF2 + 4 * (B - 3) + 2)
+            return f"s{COEF + 2 * B}", f"s{COEF + 2 * B + 1}"
+
+        self.shear_bits(HDR + 1, coef, "sh")
         self.tail("SHEARS")
+
+    def h_tsh(self, name, params, tag):
+        """a diagonal table, then the shears of the bits in the header mask"""
+        ts = self.ts
+        self.e(f"s_mov_b32 s{SCR}, s{HDR + 1}")        # shear mask
+        if name == "TSH_GEN":
+            self.e(f"s_mov_b32 s{BUF2}, s{HDR + 2}")   # table register indices (table_gen loads into HDR only)
+        if name == "TSH_FULL":
+            self.table_apply([(j, j) for j in range(self.NR)])
+            tbytes = (2 << self.R) * ts
+        elif name == "TSH_HALF":
+            (r,) = params
+            js = [j for j in range(self.NR) if (j >> r) & 1]
+            self.table_apply([(j, i) for i, j in enumerate(js)])
+            tbytes = (1 << self.R) * ts
+        else:
+            self.table_gen(BUF2, tag)
+            tbytes = (2 << self.R) * ts
+        regs = self.load_bytes(16 + tbytes, 2 * self.R * ts)
+        self.e("s_waitcnt lgkmcnt(0)")
+
+        def coef(B):
+            if self.d:
+                return sp(regs[4 * B]), sp(regs[4 * B + 2])
+            return f"s{regs[2 * B]}", f"s{regs[2 * B + 1]}"
+
+        self.shear_bits(SCR, coef, tag + "s")
+        self.tail(name)
 
     def h_table(self, name, params):
         if name == "TABLE_FULL":
@@ -240,19 +317,23 @@ class Gen:
             (r,) = params
             js = [j for j in range(self.NR) if (j >> r) & 1]
             self.table_apply([(j, i) for i, j in enumerate(js)])
-        else:  # TABLE_GEN: per-register skip on the mask (kept in s70 while loading)
-            self.e(f"s_mov_b32 s{EXEC}, s{HDR + 1}")
-            per = 4 if self.d else 8
-            for c in range((self.NR + per - 1) // per):
-                self.e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], {sp(NEXT)}, {hex(16 + 64 * c)}")
-                self.e("s_waitcnt lgkmcnt(0)")
-                for j in range(c * per, min(self.NR, (c + 1) * per)):
-                    tag = f"tg{j}"
-                    self.e(f"s_bitcmp1_b32 s{EXEC}, {j}")
-                    self.e(f"s_cbranch_scc0 {self.lbl(tag)}")
-                    self.cmul(j, self.sval(HDR, 2 * (j % per)), self.sval(HDR, 2 * (j % per) + 1))
-                    self.e(f"{self.lbl(tag)}:")
+        else:  # TABLE_GEN: per-register skip on the mask (kept in s69 while loading)
+            self.e(f"s_mov_b32 s{SCR}, s{HDR + 1}")
+            self.table_gen(SCR, "tg")
         self.tail(name)
+
+    def table_gen(self, mask_reg, tag):
+        """the table factors of the register indices set in s<mask_reg>"""
+        per = 4 if self.d else 8
+        for c in range((self.NR + per - 1) // per):
+            self.load(HDR, 16 + 64 * c)
+            self.e("s_waitcnt lgkmcnt(0)")
+            for j in range(c * per, min(self.NR, (c + 1) * per)):
+                t = f"{tag}g{j}"
+                self.e(f"s_bitcmp1_b32 s{mask_reg}, {j}")
+                self.e(f"s_cbranch_scc0 {self.lbl(t)}")
+                self.cmul(j, self.sval(HDR, 2 * (j % per)), self.sval(HDR, 2 * (j % per) + 1))
+                self.e(f"{self.lbl(t)}:")
 
     def sel(self, name, params):
         """register indices an ALL / B / BB variant acts on (None: GEN, the mask)."""
@@ -298,9 +379,9 @@ class Gen:
     def h_mat(self, params):
         (B,) = params
         f = self.f
-        if self.d:  # u11 (re, im) lies beyond the first load
-            self.e(f"s_load_dwordx4 s[{BUF2}:{BUF2 + 3}], {sp(NEXT)}, 0x40")
-            u = [self.sval(COEF, i) for i in range(6)] + [sp(BUF2), sp(BUF2 + 2)]
+        if self.d:  # u11 (re, im) lies beyond the first load (s[36:37] will hold the saved exec)
+            self.e(f"s_load_dwordx4 s[{BUF2 + 4}:{BUF2 + 7}], %[prog], s{OFF} offset:0x40")
+            u = [self.sval(COEF, i) for i in range(6)] + [sp(BUF2 + 4), sp(BUF2 + 6)]
         else:
             u = [self.sval(COEF, i) for i in range(8)]
         self.lanes_on(f"mt{B}l")
@@ -369,18 +450,13 @@ class Gen:
     # --- the whole interpreter -----------------------------------------------
     def build(self):
         e = self.e
-        e(f"s_mov_b64 {sp(NEXT)}, %[prog]")
-        e(f"{self.lbl('op')}:")
-        e(f"s_load_dwordx16 s[{HDR}:{HDR + 15}], {sp(NEXT)}, 0x0")
-        e("s_waitcnt lgkmcnt(0)")
-        # jump table: target = .Ltab + 4 * code (every entry is one s_branch)
-        e("s_getpc_b64 s[36:37]")
+        # jump table: entry = .Ltab + 4 * code (every entry is one s_branch)
+        e(f"s_mov_b32 s{OFF}, 0")
+        e(f"s_getpc_b64 s[{TAB}:{TAB + 1}]")
         e(f"{self.lbl('pc')}:")
-        e(f"s_lshl2_add_u32 s36, s{HDR}, s36")
-        e("s_addc_u32 s37, s37, 0")
-        e(f"s_add_u32 s36, s36, ({self.lbl('tab')} - {self.lbl('pc')})")
-        e("s_addc_u32 s37, s37, 0")
-        e("s_setpc_b64 s[36:37]")
+        e(f"s_add_u32 s{TAB}, s{TAB}, ({self.lbl('tab')} - {self.lbl('pc')})")
+        e(f"s_addc_u32 s{TAB + 1}, s{TAB + 1}, 0")
+        self.dispatch()
         e(f"{self.lbl('tab')}:")
         for i in range(len(self.codes)):
             e(f"s_branch {self.lbl(f'h{i}')}")
@@ -405,6 +481,8 @@ class Gen:
                 self.h_swap(params)
             elif name == "RMAT":
                 self.h_rmat(params)
+            elif name.startswith("TSH"):
+                self.h_tsh(name, params, tag)
         e(f"{self.lbl('end')}:")
         return self.L
 
