@@ -87,61 +87,71 @@ struct Regs {
 // SPLIT moves the real parts, then the imaginary parts, through an LDS array
 // of scalars: half the LDS footprint (more workgroups per CU), two more
 // barriers.  Offsets are in bytes: the thread's part xor a uniform part per
-// register (the swizzle is linear), one vector xor per access.
+// register index j (the swizzle is linear), one vector xor per access.  The
+// uniform parts xa[j] / xb[j] are formed on the host (phase_aux).
 template <int R, int W, bool SPLIT, typename T>
-__device__ __forceinline__ void exchange(Regs<T, R> &a, void *lds_raw, const dq_phase *__restrict__ A,
-                                         const dq_phase *__restrict__ B, bool lead_barrier, uint32_t tid) {
+__device__ __forceinline__ void exchange(Regs<T, R> &a, const dq_phase *__restrict__ A, const dq_phase *__restrict__ B,
+                                         const uint32_t *__restrict__ xa, const uint32_t *__restrict__ xb,
+                                         bool lead_barrier, uint32_t tid) {
     constexpr int NR = 1 << R;
     constexpr uint32_t ES = SPLIT ? sizeof(T) : 2 * sizeof(T);
     const uint32_t sa = swz(thread_tile_addr<W>(A, tid)) * ES;
     const uint32_t sb = swz(thread_tile_addr<W>(B, tid)) * ES;
-    uint32_t ra[R > 0 ? R : 1], rb[R > 0 ? R : 1];
-#pragma unroll
-    for (int r = 0; r < R; r++) { ra[r] = swz(A->reg_t[r]) * ES; rb[r] = swz(B->reg_t[r]) * ES; }
-    // register j's offset = thread part xor a uniform (scalar) part, formed at
-    // each access: one vector xor, no offset arrays held in VGPRs
-    auto offa = [&](int j) {
-        uint32_t x = 0;
-#pragma unroll
-        for (int r = 0; r < R; r++)
-            if ((j >> r) & 1) x ^= ra[r];
-        return sa ^ x;
-    };
-    auto offb = [&](int j) {
-        uint32_t x = 0;
-#pragma unroll
-        for (int r = 0; r < R; r++)
-            if ((j >> r) & 1) x ^= rb[r];
-        return sb ^ x;
-    };
-    unsigned char *lds = reinterpret_cast<unsigned char *>(lds_raw);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    auto at = [](uint32_t off) { return reinterpret_cast<T *>(lds_raw + off); };
     if (!SPLIT) {
         if (lead_barrier) __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            *reinterpret_cast<T *>(lds + offa(j)) = a.re[j];
-            *reinterpret_cast<T *>(lds + offa(j) + sizeof(T)) = a.im[j];
+            T *q = at(sa ^ xa[j]);
+            q[0] = a.re[j];
+            q[1] = a.im[j];
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            a.re[j] = *reinterpret_cast<const T *>(lds + offb(j));
-            a.im[j] = *reinterpret_cast<const T *>(lds + offb(j) + sizeof(T));
+            const T *q = at(sb ^ xb[j]);
+            a.re[j] = q[0];
+            a.im[j] = q[1];
         }
     } else {
         if (lead_barrier) __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + offa(j)) = a.re[j];
+        for (int j = 0; j < NR; j++) *at(sa ^ xa[j]) = a.re[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.re[j] = *reinterpret_cast<const T *>(lds + offb(j));
+        for (int j = 0; j < NR; j++) a.re[j] = *at(sb ^ xb[j]);
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *reinterpret_cast<T *>(lds + offa(j)) = a.im[j];
+        for (int j = 0; j < NR; j++) *at(sa ^ xa[j]) = a.im[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.im[j] = *reinterpret_cast<const T *>(lds + offb(j));
+        for (int j = 0; j < NR; j++) a.im[j] = *at(sb ^ xb[j]);
     }
+}
+
+// Per-phase and per-block records the engine's host code forms for the sweep
+// kernels (u32 arrays in the workspace):
+//   phase_aux, DQ_PAUX words per phase: [0] byte offset of the phase's device
+//     program, [1] its op count, [8 + j] the uniform part of register index j's
+//     LDS byte offset in the exchange: xor of swz(reg_t[r]) * element bytes over
+//     the bits r of j.
+//   block_aux, DQ_BAUX words per block: the non-tile qubits as runs of
+//     consecutive qubits, [i] = first bit in the tile index | run length << 16,
+//     [16 + i] = the run's lowest qubit; runs past the last have length 0.
+#define DQ_PAUX 40
+#define DQ_BAUX 32
+#define DQ_RUNS 10
+
+// Non-tile qubits of tile number t (t < 2^31) from the block's runs.
+__device__ __forceinline__ uint64_t tile_base_runs(const uint32_t *__restrict__ ba, uint32_t t) {
+    uint64_t base = 0;
+#pragma unroll
+    for (int i = 0; i < DQ_RUNS; i++) {
+        const uint32_t c = ba[i];
+        base |= (uint64_t)((t >> (c & 31u)) & ((1u << (c >> 16)) - 1u)) << ba[16 + i];
+    }
+    return base;
 }
 
 // ---- op interpreter --------------------------------------------------------
@@ -255,12 +265,14 @@ __device__ __forceinline__ bool store_needs_barrier(const dq_block *__restrict__
 // the op program of each phase in the assembly interpreter.  lead: a barrier
 // before the first exchange (LDS may still be read for a previous tile).
 template <typename T, int R, int W, bool SPLIT>
-__device__ __forceinline__ void process_tile(Regs<T, R> &a, void *lds, const dq_phase *__restrict__ ph, uint32_t nph,
-                                             const uint32_t *__restrict__ pp, const unsigned char *__restrict__ prog,
+__device__ __forceinline__ void process_tile(Regs<T, R> &a, const dq_phase *__restrict__ ph, uint32_t nph,
+                                             const uint32_t *__restrict__ pa, const unsigned char *__restrict__ prog,
                                              uint64_t cbase, uint32_t tid, bool lead) {
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W, SPLIT, T>(a, lds, ph + (p - 1), ph + p, p > 1 || lead, tid);
-        AsmOps<T, R>::run(a, prog + pp[2 * p], cbase, tid);
+        if (p > 0)
+            exchange<R, W, SPLIT, T>(a, ph + (p - 1), ph + p, pa + (p - 1) * DQ_PAUX + 8, pa + p * DQ_PAUX + 8,
+                                     p > 1 || lead, tid);
+        AsmOps<T, R>::run(a, prog + pa[p * DQ_PAUX], cbase, tid);
     }
 }
 
@@ -268,20 +280,19 @@ template <typename T, int R, int W, bool SPLIT>
 __global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R>()))
 sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
              const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
-             const uint32_t *__restrict__ pprog) {
+             const uint32_t *__restrict__ phase_aux, const uint32_t *__restrict__ block_aux) {
     using GV = __attribute__((address_space(1))) typename V2<T>::type;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     const dq_job job = jobs[blockIdx.y];
     const dq_block *__restrict__ blk = blocks + job.block;
     GV *__restrict__ st = (GV *)(job.state);
     const uint32_t tid = threadIdx.x;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t lane = tid & 63u;
-    const uint64_t base = tile_base(blk->nontile_mask, blockIdx.x);
+    const uint64_t base = tile_base_runs(block_aux + job.block * DQ_BAUX, blockIdx.x);
     Regs<T, R> a;
     load_tile<T, R, W>(a, blk, st, base, wv, lane);
-    process_tile<T, R, W, SPLIT>(a, lds_raw, phases + blk->phase_begin, blk->phase_count,
-                                 pprog + 2 * blk->phase_begin, prog, base | job.base_hi, tid, false);
+    process_tile<T, R, W, SPLIT>(a, phases + blk->phase_begin, blk->phase_count, phase_aux + blk->phase_begin * DQ_PAUX,
+                                 prog, base | job.base_hi, tid, false);
     if (store_needs_barrier<W>(blk)) __syncthreads();
     DQ_STORE_TILE(a, base);
 }
@@ -305,9 +316,8 @@ template <typename T, int R, int W, bool SPLIT>
 __global__ void __launch_bounds__(64 << W, (pipe_waves_per_eu<T, R>()))
 sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
                   const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
-                  const uint32_t *__restrict__ pprog) {
+                  const uint32_t *__restrict__ phase_aux, const uint32_t *__restrict__ block_aux) {
     using GV = __attribute__((address_space(1))) typename V2<T>::type;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     const dq_job job = jobs[blockIdx.y];
     const dq_block *__restrict__ blk = blocks + job.block;
     GV *__restrict__ st = (GV *)(job.state);
@@ -319,14 +329,15 @@ sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ 
     const uint64_t G = gridDim.x;
     const dq_phase *__restrict__ ph = phases + blk->phase_begin;
     const uint32_t nph = blk->phase_count;
-    const uint32_t *__restrict__ pp = pprog + 2 * blk->phase_begin;
+    const uint32_t *__restrict__ pa = phase_aux + blk->phase_begin * DQ_PAUX;
+    const uint32_t *__restrict__ ba = block_aux + job.block * DQ_BAUX;
     const uint64_t hi = job.base_hi;
     const bool wave_perm = store_needs_barrier<W>(blk);
     uint64_t t = blockIdx.x;
     if (t >= ntiles) return;
     Regs<T, R> a, b;
-    uint64_t ba = tile_base(msk, t);
-    load_tile<T, R, W>(a, blk, st, ba, wv, lane);
+    uint64_t ta = tile_base_runs(ba, (uint32_t)t);
+    load_tile<T, R, W>(a, blk, st, ta, wv, lane);
     bool lead = false;
     // The prefetch loads are unconditional (a tile past the end is clamped to
     // the last tile and its data left unused): a conditional load would make
@@ -334,19 +345,19 @@ sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ 
     // for the data, which turns the prefetch into a plain load.
     while (true) {
         const uint64_t t1 = t + G;
-        const uint64_t bb = tile_base(msk, t1 < ntiles ? t1 : ntiles - 1);
-        load_tile<T, R, W>(b, blk, st, bb, wv, lane);
-        process_tile<T, R, W, SPLIT>(a, lds_raw, ph, nph, pp, prog, ba | hi, tid, lead);
+        const uint64_t tb = tile_base_runs(ba, (uint32_t)(t1 < ntiles ? t1 : ntiles - 1));
+        load_tile<T, R, W>(b, blk, st, tb, wv, lane);
+        process_tile<T, R, W, SPLIT>(a, ph, nph, pa, prog, ta | hi, tid, lead);
         if (wave_perm) __syncthreads();
-        DQ_STORE_TILE(a, ba);
+        DQ_STORE_TILE(a, ta);
         lead = true;
         if (t1 >= ntiles) break;
         const uint64_t t2 = t1 + G;
-        ba = tile_base(msk, t2 < ntiles ? t2 : ntiles - 1);
-        load_tile<T, R, W>(a, blk, st, ba, wv, lane);
-        process_tile<T, R, W, SPLIT>(b, lds_raw, ph, nph, pp, prog, bb | hi, tid, lead);
+        ta = tile_base_runs(ba, (uint32_t)(t2 < ntiles ? t2 : ntiles - 1));
+        load_tile<T, R, W>(a, blk, st, ta, wv, lane);
+        process_tile<T, R, W, SPLIT>(b, ph, nph, pa, prog, tb | hi, tid, lead);
         if (wave_perm) __syncthreads();
-        DQ_STORE_TILE(b, bb);
+        DQ_STORE_TILE(b, tb);
         if (t2 >= ntiles) break;
         t = t2;
     }
@@ -991,7 +1002,8 @@ __global__ void fill_kernel(typename V2<T>::type *__restrict__ st, uint64_t N, i
 // ----------------------------------------------------------------------------
 // kernel table
 // ----------------------------------------------------------------------------
-typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, const unsigned char *, const uint32_t *);
+typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, const unsigned char *, const uint32_t *,
+                         const uint32_t *);
 
 template <typename T>
 struct SweepTable {
@@ -1224,7 +1236,31 @@ static size_t prog_bound(uint32_t n_phases, uint32_t n_ops) { return (size_t)n_o
 extern "C" size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32_t n_ops, uint32_t n_jobs) {
     return align_up(n_blocks * sizeof(dq_block), 256) + align_up(n_phases * sizeof(dq_phase), 256) +
            align_up(n_ops * sizeof(dq_op), 256) + align_up(n_jobs * sizeof(dq_job), 256) +
-           align_up(n_phases * 2 * sizeof(uint32_t), 256) + align_up(prog_bound(n_phases, n_ops), 256);
+           align_up((size_t)n_phases * DQ_PAUX * sizeof(uint32_t), 256) +
+           align_up((size_t)n_blocks * DQ_BAUX * sizeof(uint32_t), 256) + align_up(prog_bound(n_phases, n_ops), 256);
+}
+
+// Host copy of the device LDS swizzle swz().
+static uint32_t swz_host(uint32_t x) { return x ^ (((x >> 3) ^ (x >> 6) ^ (x >> 9) ^ (x >> 12)) & 7u); }
+
+// block_aux record of a block: its non-tile qubits as runs (see block_aux).
+static bool block_runs(const dq_block &b, uint32_t *out) {
+    memset(out, 0, DQ_BAUX * sizeof(uint32_t));
+    uint64_t m = b.nontile_mask;
+    uint32_t in = 0;
+    int i = 0;
+    while (m) {
+        if (i == DQ_RUNS) return false;
+        const int pos = __builtin_ctzll(m);
+        int len = 0;
+        while (pos + len < 64 && ((m >> (pos + len)) & 1)) len++;
+        out[i] = in | ((uint32_t)len << 16);
+        out[16 + i] = (uint32_t)pos;
+        in += (uint32_t)len;
+        m &= ~(((len == 64) ? ~0ull : ((1ull << len) - 1)) << pos);
+        i++;
+    }
+    return in <= 31;
 }
 
 // Device program of one phase (format and op codes in gen_ops_asm.py /
@@ -1633,22 +1669,35 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             return fail(DQ_ERR_UNSUPPORTED, "no sweep kernel for R=%d W=%d", b0.reg_bits, b0.wave_bits);
     }
 
-    // device programs of the phases of sweep blocks: (byte offset, op count) per phase
-    std::vector<uint32_t> pprog(2 * (size_t)n_phases, 0);
+    // per phase of a sweep block: its device program and exchange offsets
+    // (phase_aux); per block: the runs of its non-tile qubits (block_aux)
+    std::vector<uint32_t> paux((size_t)n_phases * DQ_PAUX, 0), baux((size_t)n_blocks * DQ_BAUX, 0);
     std::vector<int> prog_r(n_phases, -1);
+    std::vector<uint32_t> prog_es(n_phases, 0);
     std::vector<unsigned char> prog;
     for (uint32_t i = 0; i < n_blocks; i++) {
         const dq_block &b = blocks[i];
         if (b.reg_bits == 0) continue;
+        if (!block_runs(b, baux.data() + (size_t)i * DQ_BAUX))
+            return fail(DQ_ERR_UNSUPPORTED, "block %u: non-tile qubits in more than %d runs", i, DQ_RUNS);
+        const uint32_t es = (uint32_t)(use_split(prec, (int)b.tile_bits) ? elem_bytes(prec) / 2 : elem_bytes(prec));
         for (uint32_t p = b.phase_begin; p < b.phase_begin + b.phase_count; p++) {
-            if (prog_r[p] == (int)b.reg_bits) continue;
-            if (prog_r[p] >= 0) return fail(DQ_ERR_ARG, "phase %u shared by blocks of different register bits", p);
+            if (prog_r[p] == (int)b.reg_bits && prog_es[p] == es) continue;
+            if (prog_r[p] >= 0) return fail(DQ_ERR_ARG, "phase %u shared by blocks of different tile shapes", p);
             prog_r[p] = (int)b.reg_bits;
-            pprog[2 * p] = (uint32_t)prog.size();
+            prog_es[p] = es;
+            uint32_t *pa = paux.data() + (size_t)p * DQ_PAUX;
+            pa[0] = (uint32_t)prog.size();
             ProgBuilder pb(prec, (int)b.reg_bits, prog);
             const size_t start = prog.size();
-            pb.build(ops + phases[p].op_begin, phases[p].op_count, pprog[2 * p + 1]);
+            pb.build(ops + phases[p].op_begin, phases[p].op_count, pa[1]);
             if (pb.bad || !pb.check(start)) return fail(DQ_ERR_ARG, "phase %u: op not expressible as a device op", p);
+            for (uint32_t j = 0; j < (1u << b.reg_bits); j++) {
+                uint32_t x = 0;
+                for (uint32_t r = 0; r < b.reg_bits; r++)
+                    if ((j >> r) & 1) x ^= swz_host(phases[p].reg_t[r]) * es;
+                pa[8 + j] = x;
+            }
         }
     }
     if (prog.size() + 256 > prog_bound(n_phases, n_ops)) return fail(DQ_ERR_ARG, "dq_apply_blocks: device program overflow");
@@ -1658,13 +1707,15 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
     size_t off_b = 0, off_p = align_up(n_blocks * sizeof(dq_block), 256);
     size_t off_o = off_p + align_up(n_phases * sizeof(dq_phase), 256);
     size_t off_j = off_o + align_up(n_ops * sizeof(dq_op), 256);
-    size_t off_pp = off_j + align_up(n_jobs * sizeof(dq_job), 256);
-    size_t off_pg = off_pp + align_up(n_phases * 2 * sizeof(uint32_t), 256);
+    size_t off_pa = off_j + align_up(n_jobs * sizeof(dq_job), 256);
+    size_t off_ba = off_pa + align_up((size_t)n_phases * DQ_PAUX * sizeof(uint32_t), 256);
+    size_t off_pg = off_ba + align_up((size_t)n_blocks * DQ_BAUX * sizeof(uint32_t), 256);
     memcpy(buf.data() + off_b, blocks, n_blocks * sizeof(dq_block));
     memcpy(buf.data() + off_p, phases, n_phases * sizeof(dq_phase));
     if (n_ops) memcpy(buf.data() + off_o, ops, n_ops * sizeof(dq_op));
     memcpy(buf.data() + off_j, jobs, n_jobs * sizeof(dq_job));
-    if (n_phases) memcpy(buf.data() + off_pp, pprog.data(), pprog.size() * sizeof(uint32_t));
+    if (n_phases) memcpy(buf.data() + off_pa, paux.data(), paux.size() * sizeof(uint32_t));
+    memcpy(buf.data() + off_ba, baux.data(), baux.size() * sizeof(uint32_t));
     if (!prog.empty()) memcpy(buf.data() + off_pg, prog.data(), prog.size());
     buf.resize(off_pg + prog.size() + 256, 0);  // upload only what is used (+ padding)
     int rc = stage_upload(d_ws, buf, stream);
@@ -1674,7 +1725,8 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
     const dq_phase *d_phases = (const dq_phase *)(d + off_p);
     const dq_op *d_ops = (const dq_op *)(d + off_o);
     const dq_job *d_jobs = (const dq_job *)(d + off_j);
-    const uint32_t *d_pprog = (const uint32_t *)(d + off_pp);
+    const uint32_t *d_paux = (const uint32_t *)(d + off_pa);
+    const uint32_t *d_baux = (const uint32_t *)(d + off_ba);
     const unsigned char *d_prog = d + off_pg;
 
     for (uint32_t l = 0; l < n_launch; l++) {
@@ -1705,11 +1757,11 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             if (pf && resident > 0 && tiles >= 2 * resident) {
                 // persistent: every resident workgroup walks its share of the tiles
                 hipLaunchKernelGGL(pf, dim3((unsigned)resident, 1), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_pprog);
+                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_paux, d_baux);
             } else {
                 sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
                 hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_pprog);
+                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_paux, d_baux);
             }
         }
         HIPC(hipGetLastError());
