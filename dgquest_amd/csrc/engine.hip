@@ -121,12 +121,17 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, const dq_phase *__restri
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) a.re[j] = *at(sb ^ xb[j]);
+        // for R <= 4 the imaginary pass forms its 2^(R+1) addresses again (one
+        // xor each) instead of keeping them in registers across the passes; at
+        // R = 5 recomputing them spills more than keeping them
+        uint32_t sa2 = sa, sb2 = sb;
+        if constexpr (R <= 4) asm volatile("" : "+v"(sa2), "+v"(sb2));
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *at(sa ^ xa[j]) = a.im[j];
+        for (int j = 0; j < NR; j++) *at(sa2 ^ xa[j]) = a.im[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.im[j] = *at(sb ^ xb[j]);
+        for (int j = 0; j < NR; j++) a.im[j] = *at(sb2 ^ xb[j]);
     }
 }
 
