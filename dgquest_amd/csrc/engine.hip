@@ -809,8 +809,9 @@ compare_stage1(const typename V2<TA>::type *__restrict__ a, const typename V2<TB
 // v_mfma_f32_32x32x2f32: lane l supplies A[l % 32][l / 32] and
 // B[l / 32][l % 32]; result register v of lane l is
 // C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].  The host lays every real
-// matrix out so that the A operand of one MFMA is 64 consecutive floats:
-//   mat[(rb * M/2 + p) * 64 + l] = A[32 rb + l % 32][2 p + l / 32].
+// matrix out so that a lane's A operands of four consecutive MFMAs (p = 4q ..
+// 4q + 3 over the contraction pairs) are 16 contiguous bytes:
+//   mat[((rb * M/8 + q) * 64 + l) * 4 + i] = A[32 rb + l % 32][2 (4q + i) + l / 32].
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 #define DENSE_T_MAX 13
@@ -829,15 +830,18 @@ __device__ __forceinline__ uint32_t deposit_bits(uint32_t v, const uint32_t *pos
 }
 
 // One dense op on the tile in LDS.  Work item = (32 columns, RB row blocks of
-// 32); one item per wave at T = 13.  The row offsets of the B operand step
-// through the op's bits 1..k-1 with the submask increment
-// x' = ((x | ~S) + 1) & S (op bits ascending), so no table is read per MFMA.
+// 32); one item per wave at T = 13.  The B rows of the op's bits 1..k-1 step
+// with the submask increment x' = ((x | ~S) + 1) & S (op bits ascending); the
+// LDS swizzle is linear, so a B address is the lane's column part xor the
+// swizzle of x, a scalar.  The A operands of the next four MFMAs per row
+// block are loaded (16 bytes a lane) while the current four run.
 template <int KL>
 __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restrict__ im, uint32_t T,
                                          const dq_dense_op &op, const float *__restrict__ mat,
                                          uint32_t *__restrict__ coloff, uint32_t tid) {
     constexpr int K = 1 << KL, M = 2 * K, MB = M / 32;
     constexpr int RB = MB < 2 ? MB : 2;          // row blocks per work item
+    constexpr int NQ = K / 4;                    // groups of four contraction pairs (both halves)
     const uint32_t NC = 1u << (T - KL);
     const uint32_t NCH = NC / 32;
     const uint32_t items = NCH * (MB / RB);
@@ -859,46 +863,65 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
         for (int v = 0; v < 16; v++) acc[r][v] = 0.0f;
     const uint32_t item = w;
     const bool busy = item < items;  // uniform over the wave
-    uint32_t cofs = 0;
     uint32_t rg = 0;
     if (busy) {
         const uint32_t ch = item % NCH;
         rg = item / NCH;
-        cofs = coloff[ch * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u);
-        const float *__restrict__ ma = mat + (size_t)(rg * RB) * (M / 2) * 64 + lane;
-        // rows 2p + (lane / 32): p < K/2 real parts, p >= K/2 imaginary parts
-#pragma unroll 2
-        for (int half = 0; half < 2; half++) {
-            const float *__restrict__ src = half ? im : re;
-            uint32_t x = 0;
-#pragma unroll 4
-            for (int pp = 0; pp < K / 2; pp++) {
-                const int p = half * (K / 2) + pp;
-                const float bv = src[dswz(x | cofs)];
-                x = ((x | ~S) + 1u) & S;
+        const uint32_t lz = dswz(coloff[ch * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u));
+        const float4 *__restrict__ ma = reinterpret_cast<const float4 *>(mat) + (size_t)(rg * RB) * NQ * 64 + lane;
+        float4 cur[RB], nxt[RB];
 #pragma unroll
-                for (int r = 0; r < RB; r++) {
-                    const float a = ma[((size_t)r * (M / 2) + p) * 64];
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc[r], 0, 0, 0);
-                }
+        for (int r = 0; r < RB; r++) cur[r] = ma[(size_t)r * NQ * 64];
+        uint32_t x = 0;
+#pragma unroll 2
+        for (int q = 0; q < NQ; q++) {
+            if (q + 1 < NQ) {
+#pragma unroll
+                for (int r = 0; r < RB; r++) nxt[r] = ma[((size_t)r * NQ + q + 1) * 64];
             }
+            if (q == NQ / 2) x = 0;  // the imaginary half starts over
+            const float *__restrict__ src = q < NQ / 2 ? re : im;
+            float bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                bv[i] = src[dswz(x) ^ lz];
+                x = ((x | ~S) + 1u) & S;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int r = 0; r < RB; r++)
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[r][i], bv[i], acc[r], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < RB; r++) cur[r] = nxt[r];
         }
     }
     __syncthreads();  // every wave has read the whole tile: write in place
     if (busy) {
+        // Result register v of this lane is row 32 rb + 8 (v / 4) + 4 (lane / 32)
+        // + v % 4 (rb = rg RB + r); rows below K are real parts.  Its tile element
+        // is the row's bits deposited on the op's bits, or the column's bits;
+        // deposit and swizzle are linear over GF(2), so the LDS slot is the xor of
+        // sd[b] = dswz(1 << bits[b]) over the row's set bits b, xor a lane part.
         const uint32_t ch = item % NCH;
-        const uint32_t cofs0 = coloff[ch * 32 + (lane & 31u)];
-        uint32_t obits[8];
-        for (int j = 0; j < KL; j++) obits[j] = op.bits[j];
+        uint32_t sd[KL];
 #pragma unroll
-        for (int r = 0; r < RB; r++)
+        for (int b = 0; b < KL; b++) sd[b] = dswz(1u << op.bits[b]);
+        const uint32_t lp = dswz(coloff[ch * 32 + (lane & 31u)]) ^ ((lane >> 5) ? sd[2] : 0u);
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const uint32_t r0 = (rg * RB + r) * 32;  // the row block's first row (bits 5 and up)
+            uint32_t ur = 0;
+#pragma unroll
+            for (int b = 5; b < KL; b++) ur ^= ((r0 >> b) & 1u) ? sd[b] : 0u;
+            float *__restrict__ dst = (r0 >> KL) ? im : re;
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const uint32_t row = (rg * RB + r) * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const uint32_t e = dswz(deposit_bits(row & (K - 1), obits, KL) | cofs0);
-                if (row >> KL) im[e] = acc[r][v];
-                else re[e] = acc[r][v];
+                const uint32_t u = ur ^ ((v & 1) ? sd[0] : 0u) ^ ((v & 2) ? sd[1] : 0u) ^ ((v & 4) ? sd[3] : 0u) ^
+                                   ((v & 8) ? sd[4] : 0u);
+                dst[u ^ lp] = acc[r][v];
             }
+        }
     }
 }
 
@@ -918,14 +941,23 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     uint64_t hq[DENSE_T_MAX];
 #pragma unroll
     for (int b = 0; b < DENSE_T_MAX; b++) hq[b] = 1ull << blk->tile_q[b];
-    // load: element e of the tile -> qubits of its bits (bits 0..5 are qubits 0..5)
-    for (uint32_t e = tid; e < NT; e += DENSE_THREADS) {
-        uint64_t g = base | (e & 63u);
-        for (uint32_t b = 6; b < T; b++)
-            if ((e >> b) & 1u) g |= hq[b];
+    // Element e = tid + 512 i of the tile holds the amplitude whose qubits are
+    // e's bits placed on tile_q (bits 0..5 are qubits 0..5).  The placement and
+    // the LDS swizzle are linear, so the thread's part (bits 0..8) is formed
+    // once and the part of i (bits 9 and up) is a scalar.
+    uint64_t tg = base | (tid & 63u);
+#pragma unroll
+    for (int b = 6; b < 9; b++)
+        if ((tid >> b) & 1u) tg |= hq[b];
+    const uint32_t tz = dswz(tid);
+    for (uint32_t i = 0; i < NT / DENSE_THREADS; i++) {
+        uint64_t g = tg;
+        for (uint32_t b = 9; b < T; b++)
+            if ((i >> (b - 9)) & 1u) g |= hq[b];
         const typename V2<float>::type v = __builtin_nontemporal_load(st + g);
-        re[dswz(e)] = v.x;
-        im[dswz(e)] = v.y;
+        const uint32_t z = tz ^ dswz(i * DENSE_THREADS);
+        re[z] = v.x;
+        im[z] = v.y;
     }
     const size_t msz = (size_t)(2u << KL) * (2u << KL);
     for (uint32_t o = 0; o < blk->op_count; o++) {
@@ -933,13 +965,14 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
         dense_op<KL>(re, im, T, op, mats + op.mat * msz, coloff, tid);
     }
     __syncthreads();
-    for (uint32_t e = tid; e < NT; e += DENSE_THREADS) {
-        uint64_t g = base | (e & 63u);
-        for (uint32_t b = 6; b < T; b++)
-            if ((e >> b) & 1u) g |= hq[b];
+    for (uint32_t i = 0; i < NT / DENSE_THREADS; i++) {
+        uint64_t g = tg;
+        for (uint32_t b = 9; b < T; b++)
+            if ((i >> (b - 9)) & 1u) g |= hq[b];
+        const uint32_t z = tz ^ dswz(i * DENSE_THREADS);
         typename V2<float>::type v;
-        v.x = re[dswz(e)];
-        v.y = im[dswz(e)];
+        v.x = re[z];
+        v.y = im[z];
         __builtin_nontemporal_store(v, st + g);
     }
 }
@@ -2007,7 +2040,7 @@ extern "C" int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_bl
                     double a;
                     if (r < K) a = c < K ? Ure : -Uim;
                     else a = c < K ? Uim : Ure;
-                    D[((size_t)rb * (M / 2) + p) * 64 + l] = (float)a;
+                    D[(((size_t)rb * (M / 8) + p / 4) * 64 + l) * 4 + p % 4] = (float)a;
                 }
     }
     int rc = stage_upload(d_ws, buf, stream);
