@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -56,23 +57,6 @@ template <typename T> struct V2;
 template <> struct V2<double> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct V2<float> { typedef float type __attribute__((ext_vector_type(2))); };
 
-// LDS address swizzle: xor the low 3 bits of an element index with the xor of
-// its higher 3-bit groups.  A bijection, and linear over GF(2), so the swizzle
-// of a sum of disjoint bits is the xor of their swizzles.
-__device__ __forceinline__ uint32_t swz(uint32_t x) {
-    return x ^ (((x >> 3) ^ (x >> 6) ^ (x >> 9) ^ (x >> 12)) & 7u);
-}
-
-template <int W>
-__device__ __forceinline__ uint32_t thread_tile_addr(const dq_phase *__restrict__ ph, uint32_t tid) {
-    uint32_t t = 0;
-#pragma unroll
-    for (int l = 0; l < 6; l++) t |= ((tid >> l) & 1u) ? ph->lane_t[l] : 0u;
-#pragma unroll
-    for (int w = 0; w < W; w++) t |= ((tid >> (6 + w)) & 1u) ? ph->wave_t[w] : 0u;
-    return t;
-}
-
 // The register tile of one thread: 2^R amplitudes, real and imaginary parts
 // as separate scalars.  Every op below updates these registers in place, so
 // the tile keeps one register assignment through the op loop.
@@ -82,21 +66,52 @@ struct Regs {
     T im[1 << R];
 };
 
-// Re-distribute the tile from the layout of phase A to that of phase B
-// through LDS (each thread writes its 2^R amplitudes, then reads 2^R).
-// SPLIT moves the real parts, then the imaginary parts, through an LDS array
-// of scalars: half the LDS footprint (more workgroups per CU), two more
-// barriers.  Offsets are in bytes: the thread's part xor a uniform part per
-// register index j (the swizzle is linear), one vector xor per access.  The
-// uniform parts xa[j] / xb[j] are formed on the host (phase_aux).
+// Per-phase and per-block records the engine's host code forms for the sweep
+// kernels (u32 arrays in the workspace):
+//   phase_aux, DQ_PAUX words per phase: [0] byte offset of the phase's device
+//     program, [1] its op count; for the exchange from the previous phase into
+//     this one, LDS byte offsets under that exchange's swizzle:
+//     [DQ_XA + j] / [DQ_XB + j] the uniform part of register index j in the
+//     previous / this phase's layout, [DQ_TA + i] / [DQ_TB + i] the part of
+//     thread bit i (lane bits 0..5, then wave bits).
+//   block_aux, DQ_BAUX words per block: the non-tile qubits as runs of
+//     consecutive qubits, [i] = first bit in the tile index | run length << 16,
+//     [16 + i] = the run's lowest qubit; runs past the last have length 0.
+#define DQ_PAUX 96
+#define DQ_XA 8
+#define DQ_XB 40
+#define DQ_TA 72
+#define DQ_TB 81
+#define DQ_BAUX 32
+#define DQ_RUNS 10
+
+// Byte offset of a thread's part of an exchange address: xor of the parts of
+// its set thread bits.
+template <int W>
+__device__ __forceinline__ uint32_t thread_part(const uint32_t *__restrict__ t, uint32_t tid) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < 6 + W; i++) x ^= ((tid >> i) & 1u) ? t[i] : 0u;
+    return x;
+}
+
+// Re-distribute the tile from the layout of the previous phase to that of
+// this one through LDS (each thread writes its 2^R amplitudes, then reads
+// 2^R).  SPLIT moves the real parts, then the imaginary parts, through an LDS
+// array of scalars: half the LDS footprint (more workgroups per CU), two more
+// barriers.  An element's LDS slot is its tile index under a linear swizzle
+// the host picks per exchange (xor of the low bits with images of the high
+// bits) so that the lanes served together by the LDS in either layout hit
+// distinct banks; a byte offset is then the thread's part xor a uniform part
+// per register index j, both formed from the host's tables (ex = this phase's
+// phase_aux), one vector xor per access.
 template <int R, int W, bool SPLIT, typename T>
-__device__ __forceinline__ void exchange(Regs<T, R> &a, const dq_phase *__restrict__ A, const dq_phase *__restrict__ B,
-                                         const uint32_t *__restrict__ xa, const uint32_t *__restrict__ xb,
-                                         bool lead_barrier, uint32_t tid) {
+__device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restrict__ ex, bool lead_barrier, uint32_t tid) {
     constexpr int NR = 1 << R;
-    constexpr uint32_t ES = SPLIT ? sizeof(T) : 2 * sizeof(T);
-    const uint32_t sa = swz(thread_tile_addr<W>(A, tid)) * ES;
-    const uint32_t sb = swz(thread_tile_addr<W>(B, tid)) * ES;
+    const uint32_t *__restrict__ xa = ex + DQ_XA;
+    const uint32_t *__restrict__ xb = ex + DQ_XB;
+    const uint32_t sa = thread_part<W>(ex + DQ_TA, tid);
+    const uint32_t sb = thread_part<W>(ex + DQ_TB, tid);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     auto at = [](uint32_t off) { return reinterpret_cast<T *>(lds_raw + off); };
     if (!SPLIT) {
@@ -134,19 +149,6 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, const dq_phase *__restri
         for (int j = 0; j < NR; j++) a.im[j] = *at(sb2 ^ xb[j]);
     }
 }
-
-// Per-phase and per-block records the engine's host code forms for the sweep
-// kernels (u32 arrays in the workspace):
-//   phase_aux, DQ_PAUX words per phase: [0] byte offset of the phase's device
-//     program, [1] its op count, [8 + j] the uniform part of register index j's
-//     LDS byte offset in the exchange: xor of swz(reg_t[r]) * element bytes over
-//     the bits r of j.
-//   block_aux, DQ_BAUX words per block: the non-tile qubits as runs of
-//     consecutive qubits, [i] = first bit in the tile index | run length << 16,
-//     [16 + i] = the run's lowest qubit; runs past the last have length 0.
-#define DQ_PAUX 40
-#define DQ_BAUX 32
-#define DQ_RUNS 10
 
 // Non-tile qubits of tile number t (t < 2^31) from the block's runs.
 __device__ __forceinline__ uint64_t tile_base_runs(const uint32_t *__restrict__ ba, uint32_t t) {
@@ -274,9 +276,7 @@ __device__ __forceinline__ void process_tile(Regs<T, R> &a, const dq_phase *__re
                                              const uint32_t *__restrict__ pa, const unsigned char *__restrict__ prog,
                                              uint64_t cbase, uint32_t tid, bool lead) {
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0)
-            exchange<R, W, SPLIT, T>(a, ph + (p - 1), ph + p, pa + (p - 1) * DQ_PAUX + 8, pa + p * DQ_PAUX + 8,
-                                     p > 1 || lead, tid);
+        if (p > 0) exchange<R, W, SPLIT, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid);
         AsmOps<T, R>::run(a, prog + pa[p * DQ_PAUX], cbase, tid);
     }
 }
@@ -1278,8 +1278,75 @@ extern "C" size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32
            align_up((size_t)n_blocks * DQ_BAUX * sizeof(uint32_t), 256) + align_up(prog_bound(n_phases, n_ops), 256);
 }
 
-// Host copy of the device LDS swizzle swz().
-static uint32_t swz_host(uint32_t x) { return x ^ (((x >> 3) ^ (x >> 6) ^ (x >> 9) ^ (x >> 12)) & 7u); }
+// GF(2) rank of a set of bit vectors.
+static int gf2_rank(const uint32_t *v, int n) {
+    uint32_t b[32];
+    int r = 0;
+    for (int i = 0; i < n; i++) {
+        uint32_t x = v[i];
+        for (int k = 0; k < r; k++) x = std::min(x, x ^ b[k]);
+        if (x) b[r++] = x;
+    }
+    return r;
+}
+
+// LDS swizzle of one exchange: tile index x -> x ^ (xor of g[b] over the set
+// bits b >= L of x), g[b] < 2^L.  The LDS serves 128 bytes a clock, i.e. the
+// 2^L lanes whose lane bits 0..L-1 vary (2^L = 128 / element bytes); they hit
+// distinct banks iff the images of their L tile bits (bit b itself for b < L,
+// else g[b]) are linearly independent.  Picks g so that this holds for both
+// the writing layout A and the reading layout B (search over the at most 2L
+// high bits involved); false leaves g = 0 (correct, with bank conflicts).
+static bool pick_swizzle(const dq_phase &A, const dq_phase &B, int L, uint32_t *g) {
+    memset(g, 0, 32 * sizeof(uint32_t));
+    int sa[6], sb[6];
+    for (int l = 0; l < L; l++) {
+        sa[l] = __builtin_ctz(A.lane_t[l]);
+        sb[l] = __builtin_ctz(B.lane_t[l]);
+    }
+    std::vector<int> vars;
+    for (int l = 0; l < L; l++)
+        for (int b : {sa[l], sb[l]})
+            if (b >= L && std::find(vars.begin(), vars.end(), b) == vars.end()) vars.push_back(b);
+    std::vector<bool> set(32, false);
+    auto ok = [&](const int *bits) {  // assigned images of one group independent
+        uint32_t v[6];
+        int n = 0;
+        for (int l = 0; l < L; l++) {
+            const int b = bits[l];
+            if (b < L) v[n++] = 1u << b;
+            else if (set[b]) v[n++] = g[b];
+        }
+        return gf2_rank(v, n) == n;
+    };
+    std::vector<uint32_t> cand;
+    for (int i = 0; i < L; i++) cand.push_back(1u << i);
+    for (uint32_t c = 1; c < (1u << L); c++)
+        if (c & (c - 1)) cand.push_back(c);
+    long budget = 200000;
+    std::function<bool(size_t)> dfs = [&](size_t i) -> bool {
+        if (i == vars.size()) return true;
+        for (uint32_t c : cand) {
+            if (--budget < 0) return false;
+            g[vars[i]] = c;
+            set[vars[i]] = true;
+            if (ok(sa) && ok(sb) && dfs(i + 1)) return true;
+            set[vars[i]] = false;
+        }
+        g[vars[i]] = 0;
+        return false;
+    };
+    if (dfs(0)) return true;
+    memset(g, 0, 32 * sizeof(uint32_t));
+    return false;
+}
+
+// Swizzled tile index of x under images g (bits b >= L).
+static uint32_t swizzle_host(uint32_t x, const uint32_t *g, int L) {
+    uint32_t y = x;
+    for (uint32_t m = x >> L; m; m &= m - 1) y ^= g[L + __builtin_ctz(m)];
+    return y;
+}
 
 // block_aux record of a block: its non-tile qubits as runs (see block_aux).
 static bool block_runs(const dq_block &b, uint32_t *out) {
@@ -1720,21 +1787,40 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             return fail(DQ_ERR_UNSUPPORTED, "block %u: non-tile qubits in more than %d runs", i, DQ_RUNS);
         const uint32_t es = (uint32_t)(use_split(prec, (int)b.tile_bits) ? elem_bytes(prec) / 2 : elem_bytes(prec));
         for (uint32_t p = b.phase_begin; p < b.phase_begin + b.phase_count; p++) {
-            if (prog_r[p] == (int)b.reg_bits && prog_es[p] == es) continue;
-            if (prog_r[p] >= 0) return fail(DQ_ERR_ARG, "phase %u shared by blocks of different tile shapes", p);
-            prog_r[p] = (int)b.reg_bits;
-            prog_es[p] = es;
             uint32_t *pa = paux.data() + (size_t)p * DQ_PAUX;
-            pa[0] = (uint32_t)prog.size();
-            ProgBuilder pb(prec, (int)b.reg_bits, prog);
-            const size_t start = prog.size();
-            pb.build(ops + phases[p].op_begin, phases[p].op_count, pa[1]);
-            if (pb.bad || !pb.check(start)) return fail(DQ_ERR_ARG, "phase %u: op not expressible as a device op", p);
+            if (prog_r[p] != (int)b.reg_bits || prog_es[p] != es) {
+                if (prog_r[p] >= 0) return fail(DQ_ERR_ARG, "phase %u shared by blocks of different tile shapes", p);
+                prog_r[p] = (int)b.reg_bits;
+                prog_es[p] = es;
+                pa[0] = (uint32_t)prog.size();
+                ProgBuilder pb(prec, (int)b.reg_bits, prog);
+                const size_t start = prog.size();
+                pb.build(ops + phases[p].op_begin, phases[p].op_count, pa[1]);
+                if (pb.bad || !pb.check(start)) return fail(DQ_ERR_ARG, "phase %u: op not expressible as a device op", p);
+            }
+            if (p == b.phase_begin) continue;  // no exchange into a block's first phase
+            // the exchange from phase p - 1: its swizzle and offset tables
+            const dq_phase &A = phases[p - 1], &B = phases[p];
+            const int L = es == 4 ? 5 : (es == 8 ? 4 : 3);
+            uint32_t g[32];
+            pick_swizzle(A, B, L, g);
             for (uint32_t j = 0; j < (1u << b.reg_bits); j++) {
-                uint32_t x = 0;
+                uint32_t xa = 0, xb = 0;
                 for (uint32_t r = 0; r < b.reg_bits; r++)
-                    if ((j >> r) & 1) x ^= swz_host(phases[p].reg_t[r]) * es;
-                pa[8 + j] = x;
+                    if ((j >> r) & 1) {
+                        xa ^= swizzle_host(A.reg_t[r], g, L) * es;
+                        xb ^= swizzle_host(B.reg_t[r], g, L) * es;
+                    }
+                pa[DQ_XA + j] = xa;
+                pa[DQ_XB + j] = xb;
+            }
+            for (int l = 0; l < 6; l++) {
+                pa[DQ_TA + l] = swizzle_host(A.lane_t[l], g, L) * es;
+                pa[DQ_TB + l] = swizzle_host(B.lane_t[l], g, L) * es;
+            }
+            for (uint32_t w = 0; w < b.wave_bits; w++) {
+                pa[DQ_TA + 6 + w] = swizzle_host(A.wave_t[w], g, L) * es;
+                pa[DQ_TB + 6 + w] = swizzle_host(B.wave_t[w], g, L) * es;
             }
         }
     }
