@@ -816,6 +816,7 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 #define DENSE_T_MAX 13
 #define DENSE_THREADS 512
+#define DENSE_COLS 256  // column-table entries per op: 2^(T - k) <= 2^(13 - 5)
 #define DENSE_WAVES (DENSE_THREADS / 64)
 
 // LDS slot of tile element e: the low 5 bits are xored with bits 5..9 and
@@ -823,14 +824,9 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // only the low bits change, by a function of the others).
 __device__ __forceinline__ uint32_t dswz(uint32_t e) { return e ^ ((e >> 5) & 31u) ^ ((e >> 10) & 31u); }
 
-__device__ __forceinline__ uint32_t deposit_bits(uint32_t v, const uint32_t *pos, int cnt) {
-    uint32_t r = 0;
-    for (int i = 0; i < cnt; i++) r |= ((v >> i) & 1u) << pos[i];
-    return r;
-}
-
 // One dense op on the tile in LDS.  Work item = (32 columns, RB row blocks of
-// 32); one item per wave at T = 13.  The B rows of the op's bits 1..k-1 step
+// 32); one item per wave at T = 13.  coloff[c] is column c's tile offset (c's
+// bits placed on the tile bits outside the op), a table the host forms per op.  The B rows of the op's bits 1..k-1 step
 // with the submask increment x' = ((x | ~S) + 1) & S (op bits ascending); the
 // LDS swizzle is linear, so a B address is the lane's column part xor the
 // swizzle of x, a scalar.  The A operands of the next four MFMAs per row
@@ -838,7 +834,7 @@ __device__ __forceinline__ uint32_t deposit_bits(uint32_t v, const uint32_t *pos
 template <int KL>
 __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restrict__ im, uint32_t T,
                                          const dq_dense_op &op, const float *__restrict__ mat,
-                                         uint32_t *__restrict__ coloff, uint32_t tid) {
+                                         const uint32_t *__restrict__ coloff, uint32_t tid) {
     constexpr int K = 1 << KL, M = 2 * K, MB = M / 32;
     constexpr int RB = MB < 2 ? MB : 2;          // row blocks per work item
     constexpr int NQ = K / 4;                    // groups of four contraction pairs (both halves)
@@ -846,17 +842,11 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
     const uint32_t NCH = NC / 32;
     const uint32_t items = NCH * (MB / RB);
     const uint32_t w = tid >> 6, lane = tid & 63;
-    uint32_t cbits[DENSE_T_MAX];
     uint32_t used = 0;
     for (int j = 0; j < KL; j++) used |= 1u << op.bits[j];
-    int nc = 0;
-    for (uint32_t b = 0; b < T; b++)
-        if (!((used >> b) & 1u)) cbits[nc++] = b;
     const uint32_t b0 = 1u << op.bits[0];
     const uint32_t S = used & ~b0;  // tile bits of matrix-index bits 1..k-1
-    __syncthreads();  // the previous op's writes and table reads are done
-    for (uint32_t c = tid; c < NC; c += DENSE_THREADS) coloff[c] = deposit_bits(c, cbits, nc);
-    __syncthreads();
+    __syncthreads();  // the previous op's writes (or the tile load) are done
     v16f acc[RB];
 #pragma unroll
     for (int r = 0; r < RB; r++)
@@ -929,13 +919,12 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
 template <int KL>
 __global__ void __launch_bounds__(DENSE_THREADS)
 dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restrict__ ops, const float *__restrict__ mats,
-             typename V2<float>::type *__restrict__ st) {
+             const uint32_t *__restrict__ ctab, typename V2<float>::type *__restrict__ st) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t T = blk->tile_bits;
     const uint32_t NT = 1u << T;
     float *re = reinterpret_cast<float *>(lds_raw);
     float *im = re + NT;
-    uint32_t *coloff = reinterpret_cast<uint32_t *>(im + NT);
     const uint32_t tid = threadIdx.x;
     const uint64_t base = tile_base(blk->nontile_mask, blockIdx.x);
     uint64_t hq[DENSE_T_MAX];
@@ -962,7 +951,7 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     const size_t msz = (size_t)(2u << KL) * (2u << KL);
     for (uint32_t o = 0; o < blk->op_count; o++) {
         const dq_dense_op op = ops[blk->op_begin + o];
-        dense_op<KL>(re, im, T, op, mats + op.mat * msz, coloff, tid);
+        dense_op<KL>(re, im, T, op, mats + op.mat * msz, ctab + (size_t)(blk->op_begin + o) * DENSE_COLS, tid);
     }
     __syncthreads();
     for (uint32_t i = 0; i < NT / DENSE_THREADS; i++) {
@@ -2067,13 +2056,14 @@ extern "C" int dq_compare_states(const void *a, int prec_a, const void *b, int p
 // ---- dense sweeps -------------------------------------------------------------
 namespace {
 size_t dense_mat_floats(int k) { return (size_t)(2u << k) * (2u << k); }
-size_t dense_lds_bytes(uint32_t T) { return 2 * ((size_t)4 << T) + 256 * sizeof(uint32_t); }
+size_t dense_lds_bytes(uint32_t T) { return 2 * ((size_t)4 << T); }
 }  // namespace
 
 extern "C" size_t dq_dense_ws_bytes(int k, uint32_t n_blocks, uint32_t n_ops, uint32_t n_mats) {
     if (k < 5 || k > 7) return 0;
     return align_up(n_blocks * sizeof(dq_dense_block), 256) + align_up(n_ops * sizeof(dq_dense_op), 256) +
-           align_up((size_t)n_mats * dense_mat_floats(k) * sizeof(float), 256);
+           align_up((size_t)n_mats * dense_mat_floats(k) * sizeof(float), 256) +
+           align_up((size_t)n_ops * DENSE_COLS * sizeof(uint32_t), 256);
 }
 
 extern "C" int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_blocks, const dq_dense_op *ops,
@@ -2110,7 +2100,8 @@ extern "C" int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_bl
     const size_t ob = align_up(n_blocks * sizeof(dq_dense_block), 256);
     const size_t mo = ob + align_up(n_ops * sizeof(dq_dense_op), 256);
     const int K = 1 << k, M = 2 * K;
-    std::vector<unsigned char> buf(mo + (size_t)n_mats * dense_mat_floats(k) * sizeof(float), 0);
+    const size_t co = mo + align_up((size_t)n_mats * dense_mat_floats(k) * sizeof(float), 256);
+    std::vector<unsigned char> buf(co + (size_t)n_ops * DENSE_COLS * sizeof(uint32_t), 0);
     memcpy(buf.data(), blocks, n_blocks * sizeof(dq_dense_block));
     if (n_ops) memcpy(buf.data() + ob, ops, n_ops * sizeof(dq_dense_op));
     float *dst = reinterpret_cast<float *>(buf.data() + mo);
@@ -2129,21 +2120,43 @@ extern "C" int dq_apply_dense(int k, const dq_dense_block *blocks, uint32_t n_bl
                     D[(((size_t)rb * (M / 8) + p / 4) * 64 + l) * 4 + p % 4] = (float)a;
                 }
     }
+    // per op: column c's tile offset, c's bits placed on the block's tile bits
+    // outside the op (in ascending order)
+    uint32_t *ct = reinterpret_cast<uint32_t *>(buf.data() + co);
+    std::vector<int> op_T(n_ops, -1);
+    for (uint32_t i = 0; i < n_blocks; i++) {
+        const uint32_t T = blocks[i].tile_bits;
+        for (uint32_t o = blocks[i].op_begin; o < blocks[i].op_begin + blocks[i].op_count; o++) {
+            if (op_T[o] >= 0 && op_T[o] != (int)T) return fail(DQ_ERR_ARG, "dense op %u: in blocks of different tiles", o);
+            op_T[o] = (int)T;
+            uint32_t used = 0, cb[DENSE_T_MAX];
+            for (int j = 0; j < k; j++) used |= 1u << ops[o].bits[j];
+            int nc = 0;
+            for (uint32_t b = 0; b < T; b++)
+                if (!((used >> b) & 1u)) cb[nc++] = b;
+            for (uint32_t c = 0; c < (1u << nc); c++) {
+                uint32_t v = 0;
+                for (int b = 0; b < nc; b++) v |= ((c >> b) & 1u) << cb[b];
+                ct[(size_t)o * DENSE_COLS + c] = v;
+            }
+        }
+    }
     int rc = stage_upload(d_ws, buf, stream);
     if (rc) return rc;
     unsigned char *d = (unsigned char *)d_ws;
     const dq_dense_block *d_blocks = (const dq_dense_block *)d;
     const dq_dense_op *d_ops = (const dq_dense_op *)(d + ob);
     const float *d_mats = (const float *)(d + mo);
+    const uint32_t *d_ct = (const uint32_t *)(d + co);
     hipStream_t s = (hipStream_t)stream;
     for (uint32_t i = 0; i < n_blocks; i++) {
         const dq_dense_block &b = blocks[i];
         const unsigned grid = 1u << (b.n_qubits - b.tile_bits);
         const size_t lds = dense_lds_bytes(b.tile_bits);
         auto *st = (typename V2<float>::type *)state;
-        if (k == 5) hipLaunchKernelGGL(dense_kernel<5>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, st);
-        else if (k == 6) hipLaunchKernelGGL(dense_kernel<6>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, st);
-        else hipLaunchKernelGGL(dense_kernel<7>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, st);
+        if (k == 5) hipLaunchKernelGGL(dense_kernel<5>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, d_ct, st);
+        else if (k == 6) hipLaunchKernelGGL(dense_kernel<6>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, d_ct, st);
+        else hipLaunchKernelGGL(dense_kernel<7>, dim3(grid), dim3(DENSE_THREADS), lds, s, d_blocks + i, d_ops, d_mats, d_ct, st);
         HIPC(hipGetLastError());
     }
     return DQ_OK;
