@@ -14,17 +14,18 @@ namespace dgq {
 static const double kRsqrt2 = 0.7071067811865475244008443621;
 static const int kSmallMax = 6;  // n <= kSmallMax: whole-state-in-LDS kernel
 
-TuneParams fit_shape(const TuneParams &tp, int n) {
+TuneParams fit_shape(const TuneParams &tp, int n, int prec) {
     TuneParams t = tp;
     if (t.R < 1) t.R = 1;
     if (t.R > DQ_MAX_REG_BITS) t.R = DQ_MAX_REG_BITS;
     if (t.W < 0) t.W = 0;
-    if (t.W > 3) t.W = 3;
+    if (t.W > DQ_MAX_WAVE_BITS) t.W = DQ_MAX_WAVE_BITS;
     while (6 + t.R + t.W > n && t.W > 0) t.W--;
     while (6 + t.R + t.W > n && t.R > 1) t.R--;
     // instantiated shapes: W = 0 for R 1..5, W 1..2 for R 2..5, W = 3 for R 3..4
+    // (and R = 5 in FP32: the 14-bit tile)
     if (t.W >= 1 && t.R < 2) t.R = 2;
-    if (t.W == 3 && t.R > 4) t.W = 2;
+    if (t.W == 3 && t.R > 4 && prec != DQ_F32) t.W = 2;
     if (t.W == 3 && t.R < 3) t.W = 2;
     if (t.k > t.R + t.W) t.k = t.R + t.W;
     if (t.k < 1) t.k = 1;
@@ -693,7 +694,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         return;
     }
 
-    const TuneParams t = fit_shape(tp, n_local);
+    const TuneParams t = fit_shape(tp, n_local, prec);
     Dag d(n, true);
     for (const auto &g : gates) d.add(g);
     Fuse F(n);
@@ -727,7 +728,7 @@ bool plan_permutation(int n, int prec, const std::vector<int> &src, const std::v
     P.ops.clear();
     P.st = PlanStats();
     if (n <= kSmallMax) { err = "permutation pass needs n > 6"; return false; }
-    const TuneParams t = fit_shape(tp, n);
+    const TuneParams t = fit_shape(tp, n, prec);
     const int T = 6 + t.R + t.W, R = t.R, W = t.W;
     if (src.size() != dst.size()) { err = "src/dst size mismatch"; return false; }
     uint64_t moved = 0;
@@ -848,7 +849,7 @@ bool plan_restore_mapping(int n, int prec, std::vector<int> &l2p, const TunePara
         }
         return true;
     }
-    const TuneParams t = fit_shape(tp, n);
+    const TuneParams t = fit_shape(tp, n, prec);
     const int cap_high = t.R + t.W;
     for (int guard = 0; guard < 4 * n + 4 && !identity(); guard++) {
         // positions taking part in this pass: pairs (current, home) while the

@@ -75,6 +75,6 @@ bool plan_restore_mapping(int n, int prec, std::vector<int> &l2p, const TunePara
 void append_plan(Plan &dst, const Plan &src);
 
 // Clamp a tuning choice to an n-qubit state (tile = 6 + R + W <= n).
-TuneParams fit_shape(const TuneParams &tp, int n);
+TuneParams fit_shape(const TuneParams &tp, int n, int prec);
 
 }  // namespace dgq

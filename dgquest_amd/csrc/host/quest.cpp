@@ -828,7 +828,7 @@ int dgq_set_fusion(Qureg qureg, int on) {
 int dgq_last_plan_stats(Qureg qureg, int stats[8], double *flush_ms) {
     QuregState *s = st(qureg);
     if (!s || !stats) return -1;
-    const TuneParams t = fit_shape(s->tune, s->n);
+    const TuneParams t = fit_shape(s->tune, s->n, s->prec);
     stats[0] = s->last_stats.sweeps;
     stats[1] = s->last_stats.phases;
     stats[2] = s->last_stats.exchanges;
@@ -908,7 +908,7 @@ int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k,
     tp.max_ops = max_ops;
     Plan P;
     plan_circuit(gl, n, prec, DQ_INIT_LOAD, 0, 0.0, tp, P);
-    const TuneParams t = fit_shape(tp, n);
+    const TuneParams t = fit_shape(tp, n, prec);
     stats[0] = P.st.sweeps;
     stats[1] = P.st.phases;
     stats[2] = P.st.exchanges;

@@ -129,3 +129,18 @@ def test_planted_mistake_is_caught(quest, oracle):
     psi[0] = 1
     plan_model.run_plan(psi, n, blocks, phases, ops)
     assert np.abs(psi - oracle_state(oracle, n, circ)).max() > TOL
+
+
+def test_shape_fit_per_precision():
+    """Requested shapes become instantiated kernels: (8, 5, 3) keeps its
+    14-qubit tile in FP32 (an FP32-only kernel) and falls back to W = 2 in
+    FP64."""
+    from dgquest_amd import quest as Q
+
+    for prec, want in ((1, (14, 5, 3)), (2, (13, 5, 2))):
+        lib = Q.load(prec)
+        r = lib.dgq_create_recorder(20)
+        lib.apply_circuit(r, circuits.layered_u3_cz(20, 2, 1))
+        b, _, _, _ = lib.recorder_plan(r, k=8, R=5, W=3)
+        assert {(int(x["tile_bits"]), int(x["reg_bits"]), int(x["wave_bits"])) for x in b} == {want}
+        lib.dgq_destroy_recorder(r)
