@@ -829,8 +829,9 @@ __device__ __forceinline__ uint32_t dswz(uint32_t e) { return e ^ ((e >> 5) & 31
 // bits placed on the tile bits outside the op), a table the host forms per op.  The B rows of the op's bits 1..k-1 step
 // with the submask increment x' = ((x | ~S) + 1) & S (op bits ascending); the
 // LDS swizzle is linear, so a B address is the lane's column part xor the
-// swizzle of x, a scalar.  The A operands of the next four MFMAs per row
-// block are loaded (16 bytes a lane) while the current four run.
+// swizzle of x, a scalar.  The A operands (16 bytes a lane per row block)
+// and the B operands of the next four MFMAs are loaded while the current four
+// run.
 template <int KL>
 __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restrict__ im, uint32_t T,
                                          const dq_dense_op &op, const float *__restrict__ mat,
@@ -860,30 +861,38 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
         const uint32_t lz = dswz(coloff[ch * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u));
         const float4 *__restrict__ ma = reinterpret_cast<const float4 *>(mat) + (size_t)(rg * RB) * NQ * 64 + lane;
         float4 cur[RB], nxt[RB];
-#pragma unroll
-        for (int r = 0; r < RB; r++) cur[r] = ma[(size_t)r * NQ * 64];
+        float bc[4], bn[4];
         uint32_t x = 0;
-#pragma unroll 2
-        for (int q = 0; q < NQ; q++) {
-            if (q + 1 < NQ) {
-#pragma unroll
-                for (int r = 0; r < RB; r++) nxt[r] = ma[((size_t)r * NQ + q + 1) * 64];
-            }
-            if (q == NQ / 2) x = 0;  // the imaginary half starts over
+        // B rows of group q: four steps of x over the real half (q < NQ/2),
+        // then again from 0 over the imaginary half
+        auto read_b = [&](int q, float *b) {
+            if (q == NQ / 2) x = 0;
             const float *__restrict__ src = q < NQ / 2 ? re : im;
-            float bv[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                bv[i] = src[dswz(x) ^ lz];
+                b[i] = src[dswz(x) ^ lz];
                 x = ((x | ~S) + 1u) & S;
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < RB; r++) cur[r] = ma[(size_t)r * NQ * 64];
+        read_b(0, bc);
+#pragma unroll 2
+        for (int q = 0; q < NQ; q++) {
+            if (q + 1 < NQ) {  // the next group's operands load while this one's MFMAs run
+#pragma unroll
+                for (int r = 0; r < RB; r++) nxt[r] = ma[((size_t)r * NQ + q + 1) * 64];
+                read_b(q + 1, bn);
             }
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int r = 0; r < RB; r++)
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[r][i], bv[i], acc[r], 0, 0, 0);
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[r][i], bc[i], acc[r], 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < RB; r++) cur[r] = nxt[r];
+#pragma unroll
+            for (int i = 0; i < 4; i++) bc[i] = bn[i];
         }
     }
     __syncthreads();  // every wave has read the whole tile: write in place
