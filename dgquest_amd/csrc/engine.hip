@@ -1661,6 +1661,26 @@ struct ProgBuilder {
 };
 }  // namespace
 
+// The device program of one phase's ops, built on the host as
+// dq_apply_blocks builds it (no GPU needed): lets the program encoding be
+// checked against the op semantics on any host.
+extern "C" int dq_build_program(int prec, int reg_bits, const dq_op *ops, uint32_t n_ops, unsigned char *out,
+                                size_t cap, size_t *len) {
+    if ((prec != DQ_F32 && prec != DQ_F64) || reg_bits < 1 || reg_bits > DQ_MAX_REG_BITS || !len || (n_ops && !ops))
+        return fail(DQ_ERR_ARG, "dq_build_program: bad arguments");
+    std::vector<unsigned char> prog;
+    ProgBuilder pb(prec, reg_bits, prog);
+    uint32_t n_out = 0;
+    pb.build(ops, n_ops, n_out);
+    if (pb.bad || !pb.check(0)) return fail(DQ_ERR_ARG, "dq_build_program: op not expressible as a device op");
+    *len = prog.size();
+    if (out) {
+        if (cap < prog.size()) return fail(DQ_ERR_ARG, "dq_build_program: %zu bytes needed", prog.size());
+        memcpy(out, prog.data(), prog.size());
+    }
+    return DQ_OK;
+}
+
 extern "C" int dq_init_state(void *state, int n, int prec, int kind, uint64_t index, double value, void *stream) {
     if (!state || n < 1 || n > 40 || (prec != DQ_F32 && prec != DQ_F64)) return fail(DQ_ERR_ARG, "dq_init_state: bad arguments");
     if (kind != DQ_INIT_CLASSICAL && kind != DQ_INIT_PLUS) return fail(DQ_ERR_ARG, "dq_init_state: bad kind %d", kind);

@@ -192,6 +192,16 @@ int dq_apply_blocks(int prec,
 /* Bytes of device workspace dq_apply_blocks needs for these counts. */
 size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32_t n_ops, uint32_t n_jobs);
 
+/* The device program dq_apply_blocks builds for one phase's ops (register
+ * bits reg_bits), on the host with no GPU: *len gets its size; with out, up
+ * to cap bytes are written.  The format is the op interpreter's (the op-code
+ * table in the engine's generated interpreter source).  For checking the
+ * encoding on any host; it has no counterpart in the reference (the
+ * reference's fused kernel reads its gate list directly,
+ * src/cpp/QuEST_gpu_kernel.cu:175-359). */
+int dq_build_program(int prec, int reg_bits, const dq_op *ops, uint32_t n_ops, unsigned char *out, size_t cap,
+                     size_t *len);
+
 /* P(qubit q = 0) for every qubit q < n, and the total probability in
  * out[n] (n + 1 doubles, host memory).  One read of the state plus a small
  * deterministic second stage.  Synchronises `stream`.  Replaces
