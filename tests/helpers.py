@@ -214,3 +214,70 @@ def dense_run(n, circuit, state=None):
     for g in circuit:
         state = dense_gate(n, g) @ state
     return state
+
+
+# ---- tensor oracle: the state as an n-axis tensor, one gate at a time --------
+# Independent of the oracle's C loops and of the engine: each gate acts by a
+# numpy tensordot on the target axis of the sub-tensor where its controls are
+# 1 (qubit q = axis n - 1 - q of the C-ordered reshape).  Runs at 20+ qubits,
+# where the Kronecker brute force above cannot.
+def _axis(n, q):
+    return n - 1 - q
+
+
+def _apply_1q(psi, n, target, u, controls=()):
+    sl = [slice(None)] * n
+    for c in controls:
+        sl[_axis(n, c)] = 1
+    sub = psi[tuple(sl)]
+    ax = _axis(n, target) - sum(1 for c in controls if _axis(n, c) < _axis(n, target))
+    out = np.moveaxis(np.tensordot(u, sub, axes=([1], [ax])), 0, ax)
+    psi[tuple(sl)] = out
+
+
+def _phase_where_all_one(psi, n, qs, f):
+    sl = [slice(None)] * n
+    for q in qs:
+        sl[_axis(n, q)] = 1
+    psi[tuple(sl)] *= f
+
+
+def tensor_gate(psi, n, g):
+    name, qs, p = g[0], g[1], (g[2] if len(g) > 2 else ())
+    if name in ONE_Q or name in ONE_Q_R or name in ("u2Gate", "u3Gate", "unitary", "compactUnitary",
+                                                   "rotateAroundAxis"):
+        _apply_1q(psi, n, qs[0], mat1(name, p))
+    elif name in ("controlledPhaseFlip", "multiControlledPhaseFlip"):
+        _phase_where_all_one(psi, n, qs, -1.0)
+    elif name in ("controlledPhaseShift", "multiControlledPhaseShift"):
+        _phase_where_all_one(psi, n, qs, np.exp(1j * p[0]))
+    elif name == "controlledNot":
+        _apply_1q(psi, n, qs[1], mat1("pauliX", ()), (qs[0],))
+    elif name == "controlledPauliY":
+        _apply_1q(psi, n, qs[1], mat1("pauliY", ()), (qs[0],))
+    elif name in ("controlledRotateX", "controlledRotateY", "controlledRotateZ", "controlledUnitary",
+                  "controlledCompactUnitary", "controlledRotateAroundAxis"):
+        _apply_1q(psi, n, qs[1], mat1(name, p), (qs[0],))
+    elif name == "multiControlledUnitary":
+        _apply_1q(psi, n, qs[-1], mat1(name, p), tuple(qs[:-1]))
+    elif name == "swapGate":
+        a, b = _axis(n, qs[0]), _axis(n, qs[1])
+        psi[...] = np.swapaxes(psi, a, b).copy()
+    elif name == "fSim":
+        a, b = qs
+        th, phi = p
+        for sub in (("controlledPhaseShift", (a, b), (-phi,)), ("controlledNot", (a, b), ()),
+                    ("controlledRotateX", (b, a), (2 * th,)), ("controlledNot", (a, b), ())):
+            tensor_gate(psi, n, sub)
+    else:
+        raise KeyError(name)
+
+
+def tensor_run(n, circuit, state=None):
+    if state is None:
+        state = np.zeros(1 << n, dtype=complex)
+        state[0] = 1
+    psi = state.astype(complex).reshape((2,) * n).copy()
+    for g in circuit:
+        tensor_gate(psi, n, g)
+    return psi.reshape(-1)

@@ -143,6 +143,53 @@ def test_bernstein_vazirani_recovers_string(oracle):
         assert abs(oracle.prob_of_outcome(v, q, (s >> q) & 1) - 1) <= 1e-13
 
 
+@pytest.mark.parametrize("n", [3, 5, 7])
+def test_tensor_oracle_matches_dense_brute_force(n):
+    """The tensordot oracle (one axis per qubit) agrees with the Kronecker
+    brute force on random circuits over the whole gate set."""
+    rng = np.random.default_rng(300 + n)
+    circ = H.random_circuit(rng, n, 60)
+    assert np.abs(H.tensor_run(n, circ) - H.dense_run(n, circ)).max() <= 1e-12
+
+
+def test_tensor_oracle_planted_mistake():
+    """A transposed 2x2 matrix in the tensordot oracle is caught by the brute
+    force (the target-axis placement is not symmetric in the operands)."""
+    rng = np.random.default_rng(7)
+    n = 4
+    circ = [g for g in H.random_circuit(rng, n, 40, allow={"u3Gate", "controlledNot", "unitary"})]
+    good = H.tensor_run(n, circ)
+    psi = np.zeros((2,) * n, dtype=complex)
+    psi[(0,) * n] = 1
+    for g in circ:
+        if g[0] == "u3Gate":
+            H._apply_1q(psi, n, g[1][0], H.mat1("u3Gate", g[2]).T)
+        else:
+            H.tensor_gate(psi, n, g)
+    assert np.abs(psi.reshape(-1) - H.dense_run(n, circ)).max() > 1e-3
+    assert np.abs(good - H.dense_run(n, circ)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("which", ["random", "layered", "grid"])
+def test_oracle_matches_tensor_oracle_20q(oracle, which):
+    """Two independent oracles at 20 qubits: the C per-gate loop and the numpy
+    tensordot oracle, on a random circuit over the whole gate set, the
+    layered U3 + CZ workload (4 layers) and a 4 x 5 grid circuit."""
+    n = 20
+    if which == "random":
+        circ = H.random_circuit(np.random.default_rng(2020), n, 60)
+    elif which == "layered":
+        circ = circuits.layered_u3_cz(n, 4, 2)
+    else:
+        circ = circuits.grid_circuit(4, 5, 6, 4)
+    v = np.zeros(1 << n, dtype=np.complex128)
+    v[0] = 1
+    got = oracle.run(v, circ)
+    want = H.tensor_run(n, circ)
+    assert np.abs(got - want).max() <= 1e-12
+    assert abs(np.vdot(got, got).real - 1.0) <= 1e-12
+
+
 @pytest.mark.parametrize("n", [8, 12])
 def test_layered_circuit_norm_and_inverse(oracle, n):
     c = circuits.layered_u3_cz(n, 6, 9)
