@@ -3,6 +3,9 @@
 // pool with index links, explicit DFS stack instead of recursion).
 #include "dag.hpp"
 
+#include <cstdlib>
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <utility>
@@ -202,6 +205,37 @@ void Dag::remerge(int id) {
     }
 }
 
+// Node ids are a topological order: a gate's predecessors were added before
+// it, and a merge keeps the earlier node.  One pass in id order takes every
+// gate whose predecessors are taken and whose targets fit, so the target
+// budget goes to the gates nearest the front on every qubit, where the
+// depth-first walk spends it along the first front gate's chain of partners.
+void Dag::scan(uint64_t &mask, int limit) {
+    for (auto &nd : nodes_) {
+        if (!nd.alive) continue;
+        bool ready = true;
+        for (int p : nd.prev)
+            if (!nodes_[p].selected) { ready = false; break; }
+        if (ready && popc64(mask | nd.tbits) <= limit) {
+            nd.selected = true;
+            mask |= nd.tbits;
+        }
+    }
+}
+
+SelectOrder select_order() {
+    static const SelectOrder o = [] {
+        const char *e = getenv("DGQ_SELECT");
+        return (e && strcmp(e, "dfs") == 0) ? SelectOrder::DepthFirst : SelectOrder::Scan;
+    }();
+    return o;
+}
+
+bool select_order_forced() {
+    static const bool f = getenv("DGQ_SELECT") != nullptr;
+    return f;
+}
+
 void Dag::dfs(int start, uint64_t &mask, int limit) {
     // Same visiting order as a recursive depth-first walk.
     std::vector<std::pair<int, size_t>> stk;
@@ -228,7 +262,8 @@ void Dag::dfs(int start, uint64_t &mask, int limit) {
     }
 }
 
-void Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates, bool full_support) {
+void Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates, bool full_support,
+                 SelectOrder order) {
     for (auto &nd : nodes_) {
         if (!nd.alive) continue;
         nd.tbits = full_support ? nd.barrier : gate_target_bits(nd.g);
@@ -236,8 +271,12 @@ void Dag::select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_
         nd.visited = false;
         nd.selected = false;
     }
-    const std::vector<int> front = front_;
-    for (int f : front) dfs(f, target_mask, limit);
+    if (order == SelectOrder::DepthFirst) {
+        const std::vector<int> front = front_;
+        for (int f : front) dfs(f, target_mask, limit);
+    } else {
+        scan(target_mask, limit);
+    }
     // The taken gates (the first max_gates selected ones in insertion order)
     // are closed under predecessors, so on every qubit they form a prefix of
     // its chain: removing them leaves the links among the others as they are

@@ -7,11 +7,15 @@
 //             successor yet (:22-217): phase types add modulo 8 and vanish at
 //             0; phase shifts add angles; 2x2 unitaries multiply; equal X, Y,
 //             Y*, H cancel; equal rotations add half angles or cancel.
-//   select()  depth-first walk from the front nodes taking every gate whose
-//             non-diagonal targets keep popcount(targetMask) <= limit; the
-//             taken set is closed under predecessors and is emitted in
-//             insertion order, capped at max_gates; everything else goes to a
-//             new DAG (:219-259).
+//   select()  takes every gate whose non-diagonal targets keep
+//             popcount(targetMask) <= limit; the taken set is closed under
+//             predecessors and is emitted in insertion order, capped at
+//             max_gates; everything else goes to a new DAG (:219-259).  The
+//             reference visits gates depth-first from the front nodes; here
+//             the default visits them in topological (id) order, which fills
+//             the target budget from the front on every qubit and needs
+//             fewer blocks (30-qubit layered U3+CZ: 30 sweeps instead of 35).
+//             DGQ_SELECT=dfs restores the depth-first order.
 // The reference builds that new DAG by re-adding the remaining gates, which
 // also merges pairs that became mergeable after their link was made (a gate
 // that absorbed a later gate can now merge into its predecessor).  Here such
@@ -26,6 +30,13 @@
 
 namespace dgq {
 
+enum class SelectOrder { Scan, DepthFirst };
+// The default order select() visits gates in: Scan unless DGQ_SELECT=dfs.
+SelectOrder select_order();
+// Whether DGQ_SELECT fixes the order (callers that try both orders and keep
+// the better partition then use only that one).
+bool select_order_forced();
+
 class Dag {
 public:
     explicit Dag(int num_qubits, bool merge = true);
@@ -39,7 +50,7 @@ public:
     // touches (controls and diagonal gates too), for dense blocks that form
     // one matrix over their qubits.
     void select(std::vector<Gate> &out, uint64_t &target_mask, int limit, size_t max_gates,
-                bool full_support = false);
+                bool full_support = false, SelectOrder order = select_order());
 
     // All surviving gates in insertion order (a valid topological order).
     std::vector<Gate> gates() const;
@@ -60,6 +71,7 @@ private:
     void remerge(int id);
     void erase_node(int top);
     void dfs(int start, uint64_t &mask, int limit);
+    void scan(uint64_t &mask, int limit);
 
     int nq_;
     bool merge_;

@@ -115,7 +115,11 @@ bool plan_dense(const std::vector<Gate> &gates, int n, int k, DensePlan &P, std:
     while (!d.empty()) {
         std::vector<Gate> bg;
         uint64_t tm = 0x3Full;
-        d.select(bg, tm, T, (size_t)-1, true);
+        // depth-first selection: a dense block counts every qubit a gate
+        // touches, and following chains of partners packs more gates per
+        // block than the front-first scan (32-qubit grid, k = 7: 34 sweeps
+        // and 52 dense ops against 45 and 66)
+        d.select(bg, tm, T, (size_t)-1, true, SelectOrder::DepthFirst);
         if (bg.empty()) { err = "dense planning made no progress"; return false; }
         for (int q = 6; q < n && popc64(tm) < T; q++) tm |= 1ull << q;
         dq_dense_block blk;
@@ -137,7 +141,7 @@ bool plan_dense(const std::vector<Gate> &gates, int n, int k, DensePlan &P, std:
         while (!od.empty()) {
             std::vector<Gate> og;
             uint64_t om = 0;
-            od.select(og, om, k, (size_t)-1, true);
+            od.select(og, om, k, (size_t)-1, true, SelectOrder::DepthFirst);
             if (og.empty()) { err = "dense op selection made no progress"; return false; }
             // pad the support to exactly k qubits with other tile qubits
             for (int b = 0; b < T && popc64(om) < k; b++)

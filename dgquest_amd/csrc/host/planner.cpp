@@ -602,20 +602,31 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
     blk.init_value = init_value;
     blk.nontile_mask = ((1ull << nl) - 1) & ~tile;
 
-    // split the block into register phases of at most R targets each
+    // split the block into register phases of at most R targets each, in
+    // both selection orders unless one is forced; the fewer phases win (ties:
+    // the default order)
     std::vector<std::vector<Gate>> sets;
     std::vector<uint32_t> set_tbits;
     {
-        Dag pd(n, false);
-        for (const auto &g : bg) pd.add(g);
-        while (!pd.empty()) {
-            std::vector<Gate> pg;
-            uint64_t pm = 0;
-            pd.select(pg, pm, R, (size_t)-1);
-            uint32_t tb = 0;
-            for (uint64_t m = pm; m; m &= m - 1) tb |= 1u << c.q2t[ctz64(m)];
-            sets.push_back(std::move(pg));
-            set_tbits.push_back(tb);
+        auto split = [&](SelectOrder o, std::vector<std::vector<Gate>> &ss, std::vector<uint32_t> &tbs) {
+            Dag pd(n, false);
+            for (const auto &g : bg) pd.add(g);
+            while (!pd.empty()) {
+                std::vector<Gate> pg;
+                uint64_t pm = 0;
+                pd.select(pg, pm, R, (size_t)-1, false, o);
+                uint32_t tb = 0;
+                for (uint64_t m = pm; m; m &= m - 1) tb |= 1u << c.q2t[ctz64(m)];
+                ss.push_back(std::move(pg));
+                tbs.push_back(tb);
+            }
+        };
+        split(select_order(), sets, set_tbits);
+        if (!select_order_forced()) {
+            std::vector<std::vector<Gate>> s2;
+            std::vector<uint32_t> t2;
+            split(SelectOrder::DepthFirst, s2, t2);
+            if (s2.size() < sets.size()) { sets.swap(s2); set_tbits.swap(t2); }
         }
         if (sets.empty()) { sets.emplace_back(); set_tbits.push_back(0); }
     }
@@ -695,18 +706,29 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
     }
 
     const TuneParams t = fit_shape(tp, n_local, prec);
-    Dag d(n, true);
-    for (const auto &g : gates) d.add(g);
     Fuse F(n);
     F.on = tp.fuse_diagonals;
+    // fused blocks in both selection orders unless one is forced; the fewer
+    // sweeps win (ties: the default order)
     std::vector<std::vector<Gate>> bgs;
     std::vector<uint64_t> tms;
-    while (!d.empty()) {
-        std::vector<Gate> bg;
-        uint64_t tm = 0x3Full;
-        d.select(bg, tm, 6 + t.k, (size_t)t.max_ops);
-        bgs.push_back(std::move(bg));
-        tms.push_back(tm);
+    auto partition = [&](SelectOrder o, std::vector<std::vector<Gate>> &out, std::vector<uint64_t> &masks) {
+        Dag d(n, true);
+        for (const auto &g : gates) d.add(g);
+        while (!d.empty()) {
+            std::vector<Gate> bg;
+            uint64_t tm = 0x3Full;
+            d.select(bg, tm, 6 + t.k, (size_t)t.max_ops, false, o);
+            out.push_back(std::move(bg));
+            masks.push_back(tm);
+        }
+    };
+    partition(select_order(), bgs, tms);
+    if (!select_order_forced()) {
+        std::vector<std::vector<Gate>> b2;
+        std::vector<uint64_t> m2;
+        partition(SelectOrder::DepthFirst, b2, m2);
+        if (b2.size() < bgs.size()) { bgs.swap(b2); tms.swap(m2); }
     }
     if (bgs.empty() && init_kind != DQ_INIT_LOAD) {
         bgs.emplace_back();
