@@ -314,6 +314,18 @@ uint64_t Dag::front_targets() const {
     return m;
 }
 
+std::vector<size_t> Dag::first_target_use() const {
+    std::vector<size_t> first((size_t)nq_, SIZE_MAX);
+    for (size_t i = 0; i < nodes_.size(); i++) {
+        if (!nodes_[i].alive) continue;
+        for (uint64_t m = gate_target_bits(nodes_[i].g); m; m &= m - 1) {
+            const int q = ctz64(m);
+            if (q < nq_ && first[q] == SIZE_MAX) first[q] = i;
+        }
+    }
+    return first;
+}
+
 std::vector<Gate> Dag::gates() const {
     std::vector<Gate> v;
     v.reserve(alive_count_);

@@ -58,6 +58,10 @@ public:
     // Non-diagonal targets of the gates that have no predecessor.
     uint64_t front_targets() const;
 
+    // Per qubit: the smallest node id (a topological position) of a
+    // remaining gate with a non-diagonal target on it; SIZE_MAX if none.
+    std::vector<size_t> first_target_use() const;
+
 private:
     struct Node {
         Gate g;

@@ -142,6 +142,22 @@ struct Fuse {
     std::vector<int> left;  // non-diagonal gates still to come per target qubit
     cd g = 1.0;
     explicit Fuse(int n = 0) : p0(n, 1.0), p1(n, 1.0), x(n, 0), left(n, 0) {}
+    // The qubit at position a moves to to[a] (a permutation of the positions).
+    void permute(const std::vector<int> &to) {
+        std::vector<cd> a0(p0.size()), a1(p1.size());
+        std::vector<uint8_t> ax(x.size());
+        std::vector<int> al(left.size());
+        for (size_t a = 0; a < to.size() && a < p0.size(); a++) {
+            a0[to[a]] = p0[a];
+            a1[to[a]] = p1[a];
+            ax[to[a]] = x[a];
+            al[to[a]] = left[a];
+        }
+        p0.swap(a0);
+        p1.swap(a1);
+        x.swap(ax);
+        left.swap(al);
+    }
     bool trivial(int q) const { return p0[q] == cd(1.0) && p1[q] == cd(1.0) && !x[q]; }
     uint64_t flips(uint64_t mask) const {
         uint64_t f = 0;
@@ -574,8 +590,31 @@ void set_end_masks(dq_block &blk, const Layout &first, const Layout &last, const
     }
 }
 
+// Layout with the given lanes (tile bits); register bits from `prefer`'s
+// where they are not lanes, then the lowest other tile bits; waves the rest.
+Layout layout_with_lanes(const int *lane_tb, const Layout &prefer, int T, int R, int W) {
+    Layout L;
+    memset(&L, 0, sizeof L);
+    uint32_t used = 0;
+    for (int l = 0; l < 6; l++) { L.lane[l] = (uint8_t)lane_tb[l]; used |= 1u << lane_tb[l]; }
+    int nr = 0;
+    for (int r = 0; r < R; r++)
+        if (!((used >> prefer.reg[r]) & 1)) { L.reg[nr++] = prefer.reg[r]; used |= 1u << prefer.reg[r]; }
+    for (int b = 0; b < T && nr < R; b++)
+        if (!((used >> b) & 1)) { L.reg[nr++] = (uint8_t)b; used |= 1u << b; }
+    int nw = 0;
+    for (int b = 0; b < T && nw < W; b++)
+        if (!((used >> b) & 1)) { L.wave[nw++] = (uint8_t)b; used |= 1u << b; }
+    return L;
+}
+
+// One fused block.  store_pos (or null: every tile qubit stays): the qubit
+// position each tile bit is stored to, a permutation of the tile's own
+// positions that keeps 0..5 among them; the block then ends in a layout whose
+// lane l holds the tile bit stored to position l.
 void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, const TuneParams &t, int prec,
-                int init_kind, uint64_t init_index, double init_value, Fuse &F, bool final, Plan &P) {
+                int init_kind, uint64_t init_index, double init_value, Fuse &F, bool final, Plan &P,
+                const uint8_t *store_pos = nullptr) {
     const int T = 6 + t.R + t.W, R = t.R, W = t.W;
     // tile qubits: the block's targets (always containing 0..5), padded with
     // the lowest other qubits (src/cpp/QuEST_gpu_kernel.cu:505-510 pads low to
@@ -639,11 +678,28 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
         lays.push_back(L);
         opsets.push_back(&sets[i]);
     }
-    if (!lanes_identity(lays.back())) { lays.push_back(standard_layout(T, R, W)); opsets.push_back(&kNone); }
+    uint8_t store_q[DQ_MAX_TILE_BITS];
+    for (int b = 0; b < T; b++) store_q[b] = store_pos ? store_pos[b] : tile_q[b];
+    int lane_tb[6];
+    bool lanes_moved = false;
+    for (int l = 0; l < 6; l++) lane_tb[l] = -1;
+    for (int b = 0; b < T; b++)
+        if (store_q[b] < 6) lane_tb[store_q[b]] = b;
+    for (int l = 0; l < 6; l++) lanes_moved |= lane_tb[l] != l;
+    if (!lanes_moved) {
+        if (!lanes_identity(lays.back())) { lays.push_back(standard_layout(T, R, W)); opsets.push_back(&kNone); }
+    } else {
+        bool same = true;
+        for (int l = 0; l < 6; l++) same &= lays.back().lane[l] == lane_tb[l];
+        if (!same) {
+            lays.push_back(layout_with_lanes(lane_tb, lays.back(), T, R, W));
+            opsets.push_back(&kNone);
+        }
+    }
 
     blk.phase_begin = (uint32_t)P.phases.size();
     blk.phase_count = (uint32_t)lays.size();
-    set_end_masks(blk, lays.front(), lays.back(), tile_q, tile_q, R, W);
+    set_end_masks(blk, lays.front(), lays.back(), tile_q, store_q, R, W);
     OpList L;
     for (size_t p = 0; p < lays.size(); p++) {
         dq_phase ph = phase_of(lays[p], R, W);
@@ -663,8 +719,32 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
 
 }  // namespace
 
+namespace {
+
+// Positions after a block that hands lanes `from` over to `next` (masks of
+// qubits by their start-of-plan label; pos: label -> position): each qubit
+// leaving the lanes swaps positions with one entering them, both taken in
+// position order.
+void hand_over_lanes(uint64_t from, uint64_t next, std::vector<int> &pos) {
+    std::vector<int> out, in;
+    for (uint64_t m = from & ~next; m; m &= m - 1) out.push_back(ctz64(m));
+    for (uint64_t m = next & ~from; m; m &= m - 1) in.push_back(ctz64(m));
+    auto by_pos = [&](int a, int b) { return pos[a] < pos[b]; };
+    std::sort(out.begin(), out.end(), by_pos);
+    std::sort(in.begin(), in.end(), by_pos);
+    for (size_t j = 0; j < out.size() && j < in.size(); j++) std::swap(pos[out[j]], pos[in[j]]);
+}
+
+uint64_t to_positions(uint64_t labels, const std::vector<int> &pos) {
+    uint64_t m = 0;
+    for (; labels; labels &= labels - 1) m |= 1ull << pos[ctz64(labels)];
+    return m;
+}
+
+}  // namespace
+
 void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind, uint64_t init_index,
-                  double init_value, const TuneParams &tp, Plan &P, int n_local) {
+                  double init_value, const TuneParams &tp, Plan &P, int n_local, bool rotate_lanes) {
     if (n_local < 0 || n_local > n) n_local = n;
     P.blocks.clear();
     P.phases.clear();
@@ -730,6 +810,86 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         partition(SelectOrder::DepthFirst, b2, m2);
         if (b2.size() < bgs.size()) { bgs.swap(b2); tms.swap(m2); }
     }
+
+    // rotating lanes: blocks over qubit labels (positions at the start of the
+    // plan); lanes[i] = the labels at positions 0..5 during block i
+    const int T = 6 + t.R + t.W;
+    std::vector<std::vector<Gate>> rbg;
+    std::vector<uint64_t> rtile, rlanes;
+    if (rotate_lanes && n_local == n && n > T) {
+        Dag d(n, true);
+        for (const auto &g : gates) d.add(g);
+        std::vector<int> pos(n), at(n);
+        for (int q = 0; q < n; q++) pos[q] = at[q] = q;
+        uint64_t lanes = 0x3Full;
+        while (!d.empty() && rbg.size() < bgs.size()) {
+            std::vector<Gate> bg;
+            uint64_t tm = lanes;
+            d.select(bg, tm, T, (size_t)t.max_ops, false, select_order());
+            for (int p = 6; p < n && popc64(tm) < T; p++) tm |= 1ull << at[p];
+            rbg.push_back(std::move(bg));
+            rtile.push_back(tm);
+            rlanes.push_back(lanes);
+            // next lanes: the six tile qubits whose next targeted gates come
+            // first (ties: the current lanes, then the lower position)
+            const std::vector<size_t> first = d.first_target_use();
+            std::vector<int> cand;
+            for (uint64_t m = tm; m; m &= m - 1) cand.push_back(ctz64(m));
+            std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+                if (first[a] != first[b]) return first[a] < first[b];
+                const bool la = (lanes >> a) & 1, lb = (lanes >> b) & 1;
+                if (la != lb) return la;
+                return pos[a] < pos[b];
+            });
+            uint64_t next = 0;
+            for (int i = 0; i < 6; i++) next |= 1ull << cand[i];
+            hand_over_lanes(lanes, next, pos);
+            for (int q = 0; q < n; q++) at[pos[q]] = q;
+            lanes = next;
+        }
+        if (!d.empty() || rbg.size() >= bgs.size()) { rbg.clear(); rtile.clear(); rlanes.clear(); }
+    }
+
+    if (!rbg.empty()) {
+        // emit in the positions of each block; the fusion state moves with
+        // the qubits
+        for (const auto &bg : rbg)
+            for (const Gate &g : bg)
+                if (!gate_is_diagonal(g.type)) F.left[ctz64(g.tmask)]++;
+        std::vector<int> pos(n);
+        for (int q = 0; q < n; q++) pos[q] = q;
+        for (size_t i = 0; i < rbg.size(); i++) {
+            std::vector<Gate> hg = rbg[i];
+            for (Gate &g : hg) {
+                g.tmask = to_positions(g.tmask, pos);
+                g.cmask = to_positions(g.cmask, pos);
+            }
+            const uint64_t ptile = to_positions(rtile[i], pos);
+            const uint64_t next = i + 1 < rbg.size() ? rlanes[i + 1] : rlanes[i];
+            std::vector<int> pos2 = pos;
+            hand_over_lanes(rlanes[i], next, pos2);
+            std::vector<int> at(n);
+            for (int q = 0; q < n; q++) at[pos[q]] = q;
+            uint8_t store_pos[DQ_MAX_TILE_BITS];
+            int b = 0;
+            for (int p = 0; p < n; p++)
+                if ((ptile >> p) & 1) store_pos[b++] = (uint8_t)pos2[at[p]];
+            emit_block(hg, ptile, n, n_local, t, prec, i == 0 ? init_kind : DQ_INIT_LOAD, init_index, init_value, F,
+                       i + 1 == rbg.size(), P, next != rlanes[i] ? store_pos : nullptr);
+            if (next != rlanes[i]) {
+                std::vector<int> to(n);
+                for (int q = 0; q < n; q++) to[pos[q]] = pos2[q];
+                F.permute(to);
+            }
+            pos.swap(pos2);
+        }
+        bool ident = true;
+        for (int q = 0; q < n; q++) ident &= pos[q] == q;
+        if (!ident) P.moved = pos;
+        P.st.ops = (int)P.ops.size();
+        return;
+    }
+
     if (bgs.empty() && init_kind != DQ_INIT_LOAD) {
         bgs.emplace_back();
         tms.push_back(0x3Full);

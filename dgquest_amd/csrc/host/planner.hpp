@@ -45,6 +45,9 @@ struct Plan {
     std::vector<dq_phase> phases;
     std::vector<dq_op> ops;
     PlanStats st;
+    // Set when the plan moves qubits (rotating lanes): the qubit at physical
+    // position a before the plan is at moved[a] after it.  Empty otherwise.
+    std::vector<int> moved;
 };
 
 // Plan `gates` (physical qubit indices) on an n-qubit state.  The first block
@@ -55,8 +58,15 @@ struct Plan {
 // (the qubits n_local .. n-1 index the piece): every non-diagonal target must
 // be below n_local; controls and diagonals on the piece qubits become
 // workgroup conditions, resolved per piece through dq_job.base_hi.
+//
+// rotate_lanes (whole registers only): the six lane positions 0..5 may take
+// other qubits between blocks.  Each block's store moves the six tile qubits
+// whose next targeted gates come first into positions 0..5, so the next block
+// has 13 useful tile qubits rather than 7 and the six fixed ones.  The
+// partition is kept only when it needs fewer sweeps than fixed lanes; the
+// caller composes out.moved into its qubit map.
 void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind, uint64_t init_index,
-                  double init_value, const TuneParams &tp, Plan &out, int n_local = -1);
+                  double init_value, const TuneParams &tp, Plan &out, int n_local = -1, bool rotate_lanes = false);
 
 // A pass that only moves qubits: afterwards physical qubit dst[i] holds what
 // physical qubit src[i] held.  All moved qubits must fit one tile.

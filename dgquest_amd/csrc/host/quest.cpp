@@ -238,6 +238,16 @@ static void run_plan(QuregState *s, const Plan &P) {
                         s->d_ws, s->ws_cap, s->stream));
 }
 
+// Rotating lanes in whole-register plans (DGQ_ROTATE=0 turns them off, for
+// comparison).
+static bool rotate_lanes() {
+    static const bool on = [] {
+        const char *e = getenv("DGQ_ROTATE");
+        return !e || atoi(e) != 0;
+    }();
+    return on;
+}
+
 void flush(QuregState *s) {
     if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
     if (s->pending.empty() && s->init_kind == DQ_INIT_LOAD) return;
@@ -283,8 +293,12 @@ void flush(QuregState *s) {
         s->last_dense_flops = D.flops;
     } else {
         Plan P;
-        plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P);
+        plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P, -1, rotate_lanes());
         run_plan(s, P);
+        if (!P.moved.empty()) {  // the plan moved qubits: logical q now sits at moved[l2p[q]]
+            for (int q = 0; q < s->n; q++) s->l2p[q] = P.moved[s->l2p[q]];
+            for (int q = 0; q < s->n; q++) s->p2l[s->l2p[q]] = q;
+        }
         s->last_stats = P.st;
         s->last_launches = (int)P.blocks.size();
         s->last_bytes = 0;
@@ -907,7 +921,7 @@ int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k,
     tp.W = W;
     tp.max_ops = max_ops;
     Plan P;
-    plan_circuit(gl, n, prec, DQ_INIT_LOAD, 0, 0.0, tp, P);
+    plan_circuit(gl, n, prec, DQ_INIT_LOAD, 0, 0.0, tp, P, -1, rotate_lanes());
     const TuneParams t = fit_shape(tp, n, prec);
     stats[0] = P.st.sweeps;
     stats[1] = P.st.phases;
@@ -953,12 +967,12 @@ int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_bl
     tp.max_ops = max_ops;
     tp.fuse_diagonals = fuse != 0;
     Plan P;
-    plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, tp, P);
+    plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, tp, P, -1, rotate_lanes());
     counts[0] = (int)P.blocks.size();
     counts[1] = (int)P.phases.size();
     counts[2] = (int)P.ops.size();
-    if (l2p)
-        for (int i = 0; i < s->n; i++) l2p[i] = s->l2p[i];
+    if (l2p)  // the map after the plan, as a flush would leave it
+        for (int i = 0; i < s->n; i++) l2p[i] = P.moved.empty() ? s->l2p[i] : P.moved[s->l2p[i]];
     if (counts[0] > cap_blocks || counts[1] > cap_phases || counts[2] > cap_ops) return 1;
     if (counts[0]) memcpy(blocks, P.blocks.data(), P.blocks.size() * sizeof(dq_block));
     if (counts[1]) memcpy(phases, P.phases.data(), P.phases.size() * sizeof(dq_phase));

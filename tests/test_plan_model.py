@@ -144,3 +144,30 @@ def test_shape_fit_per_precision():
         b, _, _, _ = lib.recorder_plan(r, k=8, R=5, W=3)
         assert {(int(x["tile_bits"]), int(x["reg_bits"]), int(x["wave_bits"])) for x in b} == {want}
         lib.dgq_destroy_recorder(r)
+
+
+@pytest.mark.parametrize("which", ["layered", "random", "ghz_qft", "grid"])
+@pytest.mark.parametrize("shape", [(4, 4, 0, 4096), (5, 4, 1, 4096), (6, 3, 3, 4096)])
+def test_rotating_lanes(quest, oracle, which, shape):
+    """Registers wider than the tile: blocks hand lane positions 0..5 over to
+    the qubits needed next (stores that move qubits, the fusion state moving
+    with them).  The plan ends with a permuted qubit map and matches the
+    oracle once read through it."""
+    n = 16 if which != "grid" else 15
+    if which == "layered":
+        circ = circuits.layered_u3_cz(n, 8, 3)
+    elif which == "random":
+        circ = H.random_circuit(np.random.default_rng(77), n, 200)
+    elif which == "ghz_qft":
+        circ = circuits.ghz_qft(n)
+    else:
+        circ = circuits.grid_circuit(3, 5, 10, 4)
+    k, R, W, mo = shape
+    q = quest.dgq_create_recorder(n)
+    quest.apply_circuit(q, circ)
+    _, _, _, l2p = quest.recorder_plan(q, k, R, W, mo, True)
+    quest.dgq_destroy_recorder(q)
+    assert l2p != list(range(n))  # lanes were handed over
+    psi, _ = plan_and_run(quest, n, circ, k=k, R=R, W=W, max_ops=mo)
+    ref = oracle_state(oracle, n, circ)
+    assert np.abs(psi - ref).max() <= TOL
