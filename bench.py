@@ -13,6 +13,12 @@ sweep moves 2 x 16 x 2^30 bytes (one read and one write of every amplitude)
 and the first sweep of a step, which generates |0...0> instead of reading,
 moves half of that.
 
+The autotuner times every candidate shape with lane rotation on and off and
+keeps the least GPU time per step.  Rotation (the default for API flushes)
+needs fewer sweeps per circuit, each doing more gates, so it lowers GB/s while
+it shortens the step; config.fixed_lanes_best gives the best fixed-lane plan
+of the same run (qubits 0..5 in every tile, as in the reference's blocks).
+
 roofline: the sweep kernel's algorithmic bytes per launch over its average
 launch duration, timed with HIP events on the register's own stream.
 cpu_baseline: the CPU oracle (oracle/oracle.c, OpenMP, gate by gate) on a
@@ -333,20 +339,31 @@ def main():
         return eng.elapsed_ms(e0, e1)
 
     # ---- autotuner: fusion shape with the least GPU time for this circuit
+    # every shape with lane rotation on and off (fixed lanes keep qubits 0..5
+    # in every tile, as the reference's blocks do); the least GPU time wins
     tuning = {}
     if args.shape:
         k, R, W = (int(x) for x in args.shape.split(","))
-        best = (k, R, W)
+        best = (k, R, W, 1)
         q64.dgq_set_tuning(q, k, R, W, 100000)
     else:
         for (k, R, W) in CANDIDATES:
-            q64.dgq_set_tuning(q, k, R, W, 100000)
-            gpu_time_of_step()  # first run of a shape warms it
-            ms = gpu_time_of_step()
-            st = q64.plan_stats(q)
-            tuning[f"k{k}_R{R}_W{W}"] = {"ms": round(ms, 2), "sweeps": st["sweeps"]}
-        best = min(CANDIDATES, key=lambda c: tuning[f"k{c[0]}_R{c[1]}_W{c[2]}"]["ms"])
+            for rot in (1, 0):
+                q64.dgq_set_tuning(q, k, R, W, 100000)
+                q64.dgq_set_lane_rotation(q, rot)
+                gpu_time_of_step()  # first run of a shape warms it
+                ms = gpu_time_of_step()
+                st = q64.plan_stats(q)
+                step_bytes_c, _ = q64.flush_bytes(q)
+                tuning[f"k{k}_R{R}_W{W}" + ("" if rot else "_fixed_lanes")] = {
+                    "ms": round(ms, 2), "sweeps": st["sweeps"],
+                    "hbm_GBps": round(step_bytes_c / (ms * 1e-3) / 1e9, 1)}
+        key = lambda c: f"k{c[0]}_R{c[1]}_W{c[2]}" + ("" if c[3] else "_fixed_lanes")
+        combos = [(k, R, W, rot) for (k, R, W) in CANDIDATES for rot in (1, 0)]
+        best = min(combos, key=lambda c: tuning[key(c)]["ms"])
+        best_fixed = min([c for c in combos if not c[3]], key=lambda c: tuning[key(c)]["ms"])
         q64.dgq_set_tuning(q, best[0], best[1], best[2], 100000)
+    q64.dgq_set_lane_rotation(q, best[3])
 
     # ---- warmup
     for _ in range(args.warmup):
@@ -401,6 +418,7 @@ def main():
             "fusion_k": best[0],
             "tile_R": best[1],
             "tile_W": best[2],
+            "lane_rotation": bool(best[3]),
             "sweeps_per_step": st["sweeps"],
             "lds_exchanges_per_step": st["exchanges"],
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
@@ -418,6 +436,13 @@ def main():
             "avg_launch_ms": round(gpu_ms / (args.steps * max(1, launches)), 4),
         },
     }
+    if not args.shape:
+        # the best plan that keeps qubits 0..5 in every tile, from the same
+        # autotuning run: fewer gates per sweep, more sweeps per circuit
+        fb = tuning[f"k{best_fixed[0]}_R{best_fixed[1]}_W{best_fixed[2]}_fixed_lanes"]
+        out["config"]["fixed_lanes_best"] = {
+            "shape": f"k{best_fixed[0]}_R{best_fixed[1]}_W{best_fixed[2]}", "gpu_ms_per_step": fb["ms"],
+            "sweeps": fb["sweeps"], "hbm_GBps": fb["hbm_GBps"], "frac": round(fb["hbm_GBps"] / HBM_PEAK_GBS, 4)}
     # serving first among the extra legs: its plain loop allocates and frees a
     # register per request, and device allocations became ~10x slower for
     # small registers after the precision leg's 64 + 3 x 32 GiB registers had

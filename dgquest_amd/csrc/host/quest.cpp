@@ -248,6 +248,8 @@ static bool rotate_lanes() {
     return on;
 }
 
+static bool rotation_for(const QuregState *s) { return s->rotate >= 0 ? s->rotate != 0 : rotate_lanes(); }
+
 void flush(QuregState *s) {
     if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
     if (s->pending.empty() && s->init_kind == DQ_INIT_LOAD) return;
@@ -293,7 +295,8 @@ void flush(QuregState *s) {
         s->last_dense_flops = D.flops;
     } else {
         Plan P;
-        plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P, -1, rotate_lanes());
+        plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P, -1,
+                     rotation_for(s));
         run_plan(s, P);
         if (!P.moved.empty()) {  // the plan moved qubits: logical q now sits at moved[l2p[q]]
             for (int q = 0; q < s->n; q++) s->l2p[q] = P.moved[s->l2p[q]];
@@ -831,6 +834,14 @@ int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops) {
     return 0;
 }
 
+int dgq_set_lane_rotation(Qureg qureg, int on) {
+    QuregState *s = st(qureg);
+    if (!s) return -1;
+    flush(s);
+    s->rotate = on != 0;
+    return 0;
+}
+
 int dgq_set_fusion(Qureg qureg, int on) {
     QuregState *s = st(qureg);
     if (!s) return -1;
@@ -967,7 +978,7 @@ int dgq_recorder_plan(Qureg q, int k, int R, int W, int max_ops, int fuse, dq_bl
     tp.max_ops = max_ops;
     tp.fuse_diagonals = fuse != 0;
     Plan P;
-    plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, tp, P, -1, rotate_lanes());
+    plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, tp, P, -1, rotation_for(s));
     counts[0] = (int)P.blocks.size();
     counts[1] = (int)P.phases.size();
     counts[2] = (int)P.ops.size();

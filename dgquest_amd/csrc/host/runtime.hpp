@@ -43,6 +43,7 @@ struct QuregState {
     double last_flush_ms = 0.0;
     bool recorder = false;            // records and plans only, no device state
     int dense_k = 0;                  // > 0: flushes run as dense sweeps of this width (FP32)
+    int rotate = -1;                  // lane rotation in plans: 1 on, 0 off, -1 the default (on; DGQ_ROTATE)
     double last_dense_flops = 0.0;    // device GEMM flops of the last dense flush
     dgq::PagedStore *paged = nullptr; // state in pinned host pieces, paged through the device
 };

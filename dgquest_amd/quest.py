@@ -153,6 +153,8 @@ class QuEST:
         L.dgq_last_plan_stats.argtypes = [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.dgq_set_tuning.restype = C.c_int
         L.dgq_set_tuning.argtypes = [Qureg, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.dgq_set_lane_rotation.restype = C.c_int
+        L.dgq_set_lane_rotation.argtypes = [Qureg, C.c_int]
         L.dgq_set_fusion.restype = C.c_int
         L.dgq_set_fusion.argtypes = [Qureg, C.c_int]
         L.dgq_stream.restype = C.c_void_p

@@ -23,6 +23,13 @@ extern "C" {
  * clamped to what the register size and the built kernels allow. */
 int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops);
 
+/* Lane rotation on (1, default) or off (0) for this register's flushes:
+ * with it, each fused sweep's store hands the six coalesced qubit positions
+ * to the qubits the next sweep needs, which takes fewer sweeps per circuit
+ * (each doing more gates); the qubit map absorbs the moves.  Off, those six
+ * positions keep their qubits, as in the reference's blocks. */
+int dgq_set_lane_rotation(Qureg qureg, int on);
+
 /* Diagonal fusion on (1, default) or off (0): with it, uncontrolled
  * single-qubit gates run as real 2x2 rotations and their diagonal parts are
  * deferred and merged (fewer floating-point operations per gate). */
