@@ -837,7 +837,7 @@ int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops) {
 int dgq_set_lane_rotation(Qureg qureg, int on) {
     QuregState *s = st(qureg);
     if (!s) return -1;
-    flush(s);
+    if (!s->recorder) flush(s);  // a recording register only plans: nothing to flush
     s->rotate = on != 0;
     return 0;
 }

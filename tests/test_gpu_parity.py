@@ -268,3 +268,32 @@ def test_tile_shapes_fp32(quest32, env32, oracle, shape):
     print(f"fp32 shape={shape} err={err:.3e} fid={fid:.9f} stats={st}", flush=True)
     assert err <= 1e-5 and fid >= 1 - 1e-6
     quest32.destroyQureg(q, env32)
+
+
+@pytest.mark.parametrize("rot", [1, 0])
+@pytest.mark.parametrize("n,which", [(16, "random"), (18, "layered"), (16, "grid")])
+def test_lane_rotation_on_off(quest, env, oracle, rot, n, which):
+    """Registers wider than the tile (12 qubits here): flushes whose sweeps
+    hand lane positions 0..5 over to other qubits (rotation on) and flushes
+    that keep qubits 0..5 in every tile (off) both match the oracle, read
+    through the register's qubit map; measurement outcomes from one seed then
+    match too, and so does the collapsed state."""
+    if which == "random":
+        circ = H.random_circuit(np.random.default_rng(600 + n), n, 200)
+    elif which == "layered":
+        circ = circuits.layered_u3_cz(n, 6, 5)
+    else:
+        circ = circuits.grid_circuit(4, 4, 10, 4)
+    q = quest.createQureg(n, env)
+    quest.dgq_set_tuning(q, 6, 3, 3, 4096)
+    quest.dgq_set_lane_rotation(q, rot)
+    quest.apply_circuit(q, circ)
+    ref = run_oracle(oracle, n, circ)
+    assert np.abs(quest.state_vector(q) - ref).max() <= TOL
+    keys = [n, rot, 7]
+    quest.seed(keys)
+    oracle.seed(keys)
+    for t in [int(x) for x in np.random.default_rng(n).permutation(n)][:6]:
+        assert quest.measure(q, t) == oracle.measure(ref, t)[0]
+    assert np.abs(quest.state_vector(q) - ref).max() <= TOL
+    quest.destroyQureg(q, env)

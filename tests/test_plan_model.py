@@ -171,3 +171,32 @@ def test_rotating_lanes(quest, oracle, which, shape):
     psi, _ = plan_and_run(quest, n, circ, k=k, R=R, W=W, max_ops=mo)
     ref = oracle_state(oracle, n, circ)
     assert np.abs(psi - ref).max() <= TOL
+
+
+def test_lane_rotation_switch(quest, oracle):
+    """dgq_set_lane_rotation(q, 0) keeps qubits 0..5 in every tile: the plan
+    leaves the qubit map alone and needs at least as many sweeps as the
+    rotating plan; both match the oracle."""
+    from tests.plan_model import run_plan
+
+    n = 16
+    circ = circuits.layered_u3_cz(n, 8, 3)
+    ref = oracle_state(oracle, n, circ)
+    sweeps = {}
+    for rot in (0, 1):
+        q = quest.dgq_create_recorder(n)
+        quest.dgq_set_lane_rotation(q, rot)
+        quest.apply_circuit(q, circ)
+        blocks, phases, ops, l2p = quest.recorder_plan(q, 6, 3, 3, 4096, True)
+        quest.dgq_destroy_recorder(q)
+        psi = np.zeros(1 << n, dtype=np.complex128)
+        psi[0] = 1.0
+        run_plan(psi, n, blocks, phases, ops)
+        idx = np.arange(1 << n, dtype=np.int64)
+        phys = np.zeros_like(idx)
+        for lq in range(n):
+            phys |= ((idx >> lq) & 1) << l2p[lq]
+        assert np.abs(psi[phys] - ref).max() <= TOL
+        assert (l2p == list(range(n))) == (rot == 0)
+        sweeps[rot] = len(blocks)
+    assert sweeps[1] < sweeps[0]
