@@ -691,6 +691,16 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
     } else {
         bool same = true;
         for (int l = 0; l < 6; l++) same &= lays.back().lane[l] == lane_tb[l];
+        // the last phase can take the store's lanes itself when it targets
+        // none of them and is not the block's first (whose lanes the load
+        // fixes); otherwise one more exchange, into a phase without ops
+        uint32_t lane_set = 0;
+        for (int l = 0; l < 6; l++) lane_set |= 1u << lane_tb[l];
+        const uint32_t last_targets = set_tbits.empty() ? 0u : set_tbits.back();
+        if (!same && lays.size() > 1 && opsets.back() != &kNone && !(last_targets & lane_set)) {
+            lays.back() = layout_with_lanes(lane_tb, lays.back(), T, R, W);
+            same = true;
+        }
         if (!same) {
             lays.push_back(layout_with_lanes(lane_tb, lays.back(), T, R, W));
             opsets.push_back(&kNone);
