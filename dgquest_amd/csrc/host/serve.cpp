@@ -287,10 +287,6 @@ TuneParams tune_of(const dgs_config &c, int prec, int n) {
 // The request's circuit, then the qubit moves that undo its relabelling
 // swaps: samples walk the CDF in logical index order, so the final state must
 // be stored in logical order.
-//
-// Registers wider than the tile are also planned with rotating lanes; the
-// moves then join the restore (the samples need logical order), and the plan
-// with fewer sweeps in all is kept (ties: fixed lanes).
 void plan_one(const Req &r, int prec, const TuneParams &tp, Plan &out) {
     std::vector<int> l2p = r.l2p;
     std::vector<Plan> passes;
@@ -305,22 +301,6 @@ void plan_one(const Req &r, int prec, const TuneParams &tp, Plan &out) {
         plan_circuit(g, r.n, prec, r.init_kind, r.init_index, r.init_value, tp, out);
     }
     for (const Plan &p : passes) append_plan(out, p);
-    if (!extra.empty()) return;  // small registers: one tile, nothing to rotate
-
-    Plan rot;
-    plan_circuit(r.gates, r.n, prec, r.init_kind, r.init_index, r.init_value, tp, rot, -1, true);
-    if (rot.moved.empty() || rot.blocks.size() >= out.blocks.size()) return;
-    std::vector<int> l2p2 = r.l2p;
-    for (int q = 0; q < r.n; q++) l2p2[q] = rot.moved[l2p2[q]];
-    std::vector<Plan> passes2;
-    std::vector<Gate> extra2;
-    if (!plan_restore_mapping(r.n, prec, l2p2, tp, passes2, extra2, err) || !extra2.empty()) return;
-    size_t total = rot.blocks.size();
-    for (const Plan &p : passes2) total += p.blocks.size();
-    if (total >= out.blocks.size()) return;
-    rot.moved.clear();
-    for (const Plan &p : passes2) append_plan(rot, p);
-    out = std::move(rot);
 }
 
 void plan_req(Req &r, int prec, const dgs_config &cfg) {
