@@ -835,7 +835,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         while (!d.empty() && rbg.size() < bgs.size()) {
             std::vector<Gate> bg;
             uint64_t tm = lanes;
-            d.select(bg, tm, T, (size_t)t.max_ops, false, select_order());
+            d.select(bg, tm, 6 + t.k, (size_t)t.max_ops, false, select_order());
             for (int p = 6; p < n && popc64(tm) < T; p++) tm |= 1ull << at[p];
             rbg.push_back(std::move(bg));
             rtile.push_back(tm);
