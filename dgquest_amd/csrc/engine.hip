@@ -138,9 +138,11 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restri
         for (int j = 0; j < NR; j++) a.re[j] = *at(sb ^ xb[j]);
         // for R <= 4 the imaginary pass forms its 2^(R+1) addresses again (one
         // xor each) instead of keeping them in registers across the passes; at
-        // R = 5 recomputing them spills more than keeping them
+        // R = 5 recomputing them spills more than keeping them.  FP64 tiles of
+        // 8 waves keep them: two workgroups per CU (the LDS bound) already
+        // limit those to 4 waves per SIMD, so the extra registers are free
         uint32_t sa2 = sa, sb2 = sb;
-        if constexpr (R <= 4) asm volatile("" : "+v"(sa2), "+v"(sb2));
+        if constexpr (R <= 4 && !(std::is_same<T, double>::value && W == 3)) asm volatile("" : "+v"(sa2), "+v"(sb2));
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) *at(sa2 ^ xa[j]) = a.im[j];
