@@ -297,3 +297,27 @@ def test_lane_rotation_on_off(quest, env, oracle, rot, n, which):
         assert quest.measure(q, t) == oracle.measure(ref, t)[0]
     assert np.abs(quest.state_vector(q) - ref).max() <= TOL
     quest.destroyQureg(q, env)
+
+
+@pytest.mark.parametrize("tune", [None, (6, 3, 3, 4096)])
+def test_random_22q_default_plan(quest, env, oracle, tune):
+    """A 22-qubit random circuit over the whole gate set (relabelling swaps,
+    fSim, multi-controlled gates) with the register's default plan (lane
+    rotation on; the tuning table's shape, or a 12-qubit tile): amplitudes
+    within 1e-12 of the oracle, then seeded measurements and the collapsed
+    state."""
+    n = 22
+    circ = H.random_circuit(np.random.default_rng(2222), n, 300)
+    q = quest.createQureg(n, env)
+    if tune is not None:
+        quest.dgq_set_tuning(q, *tune)
+    quest.apply_circuit(q, circ)
+    ref = run_oracle(oracle, n, circ)
+    assert np.abs(quest.state_vector(q) - ref).max() <= TOL
+    keys = [22, 5]
+    quest.seed(keys)
+    oracle.seed(keys)
+    for t in (21, 0, 13, 6):
+        assert quest.measure(q, t) == oracle.measure(ref, t)[0]
+    assert np.abs(quest.state_vector(q) - ref).max() <= TOL
+    quest.destroyQureg(q, env)
