@@ -200,3 +200,34 @@ def test_lane_rotation_switch(quest, oracle):
         assert (l2p == list(range(n))) == (rot == 0)
         sweeps[rot] = len(blocks)
     assert sweeps[1] < sweeps[0]
+
+
+@pytest.mark.parametrize("which,n", [("layered", 30), ("hea", 30), ("ghz_qft", 30), ("grid", 28), ("random", 24),
+                                     ("bv", 31)])
+def test_rotation_never_adds_sweeps(quest, which, n):
+    """Plans only (no execution) of the workload families at their benchmark
+    sizes: with lane rotation on, never more sweeps than with it off, and
+    strictly fewer on the layered, grid and random circuits."""
+    if which == "layered":
+        circ = circuits.layered_u3_cz(n, 24, 2)
+    elif which == "hea":
+        circ = circuits.hea(n, 10, 7)
+    elif which == "ghz_qft":
+        circ = circuits.ghz_qft(n)
+    elif which == "grid":
+        circ = circuits.grid_circuit(4, 7, 20, 4)
+    elif which == "random":
+        circ = H.random_circuit(np.random.default_rng(5), n, 600)
+    else:
+        circ = circuits.bernstein_vazirani(n - 1, 3)[0]
+    sweeps = {}
+    for rot in (0, 1):
+        q = quest.dgq_create_recorder(n)
+        quest.dgq_set_lane_rotation(q, rot)
+        quest.apply_circuit(q, circ)
+        blocks, _, _, _ = quest.recorder_plan(q, 7, 4, 3, 4096, True)
+        quest.dgq_destroy_recorder(q)
+        sweeps[rot] = len(blocks)
+    assert sweeps[1] <= sweeps[0]
+    if which in ("layered", "grid", "random"):
+        assert sweeps[1] < sweeps[0]
