@@ -661,7 +661,7 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
             }
         };
         split(select_order(), sets, set_tbits);
-        if (!select_order_forced()) {
+        if (!select_order_forced() && sets.size() > 1) {  // one phase cannot be beaten
             std::vector<std::vector<Gate>> s2;
             std::vector<uint32_t> t2;
             split(SelectOrder::DepthFirst, s2, t2);
@@ -814,7 +814,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
         }
     };
     partition(select_order(), bgs, tms);
-    if (!select_order_forced()) {
+    if (!select_order_forced() && bgs.size() > 1) {  // one block cannot be beaten
         std::vector<std::vector<Gate>> b2;
         std::vector<uint64_t> m2;
         partition(SelectOrder::DepthFirst, b2, m2);
@@ -826,7 +826,7 @@ void plan_circuit(const std::vector<Gate> &gates, int n, int prec, int init_kind
     const int T = 6 + t.R + t.W;
     std::vector<std::vector<Gate>> rbg;
     std::vector<uint64_t> rtile, rlanes;
-    if (rotate_lanes && n_local == n && n > T) {
+    if (rotate_lanes && n_local == n && n > T && bgs.size() > 1) {
         Dag d(n, true);
         for (const auto &g : gates) d.add(g);
         std::vector<int> pos(n), at(n);
