@@ -91,6 +91,25 @@ def traffic_from_profiles():
     return None
 
 
+def host_info():
+    """The host the bench ran on: CPU model, logical CPUs, CPUs this process
+    may use, and total RAM (from /proc)."""
+    model, mem_kib = None, None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal:"):
+                mem_kib = int(line.split()[1])
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+            "mem_total_GiB": round(mem_kib / (1 << 20), 1) if mem_kib else None}
+
+
 def cpu_baseline(circ, n, seconds):
     """Oracle gate by gate on a 2^n FP64 state until `seconds` have passed."""
     import numpy as np
@@ -194,6 +213,9 @@ def serving(q64, args):
     rs = S.RequestSet(recs, [1024] * len(mix), [d["shot_seed"] for d in mix])
     arrive = [d["arrive_s"] for d in mix]
     plain = S.run_plain(q64, rs, arrive)
+    # the same one-at-a-time loop with FP32 states: separates what batching
+    # and overlap gain from what the lower precision gains
+    plain32 = S.run_plain(q64, rs, arrive, prec=S.DQ_F32)
     budget = int(args.serve_budget_gb * 1e9)
     with S.Server(q64, prec=S.DQ_F32, hbm_budget=budget, recheck_fraction=0.01) as srv:
         served = srv.replay(rs, arrive)
@@ -204,21 +226,32 @@ def serving(q64, args):
         recs2 = [S.record(q64, d["n"], circuits.mix_circuit(d)) for d in mix2]
         rs2 = S.RequestSet(recs2, [1024] * len(mix2), [d["shot_seed"] for d in mix2])
         at_load = srv.replay(rs2, [d["arrive_s"] for d in mix2])
+        # accuracy pass (not timed): the FP64 recheck forced on a spread of
+        # the same requests -- every 20th, plus the five largest registers
+        pick = sorted(set(range(0, len(mix), 20)) |
+                      set(sorted(range(len(mix)), key=lambda i: (-mix[i]["n"], i))[:5]))
+        rs3 = S.RequestSet([recs[i] for i in pick], [1024] * len(pick), [mix[i]["shot_seed"] for i in pick],
+                           [S.DGS_RECHECK] * len(pick))
+        checked = srv.replay(rs3)
     if args.serve_dump:
         with open(args.serve_dump, "w") as f:
             for tag, rows, mx in (("plain", plain, mix), ("served", served, mix), ("at_load", at_load, mix2)):
                 for d, r in zip(mx, rows):
                     f.write(json.dumps({"run": tag, "kind": d["kind"], "n": d["n"], "depth": d["depth"],
                                         **{k: r[k] for k in ("status", "t_arrive_ms", "t_start_ms", "t_done_ms",
-                                                             "batch_size", "rechecked", "fidelity", "sum_z",
-                                                             "shot_mismatches")}}) + "\n")
-    bad = [r for r in plain + served + at_load if r["status"] != 0]
+                                                             "batch_size", "rechecked", "fidelity", "fidelity_raw",
+                                                             "total_prob", "sum_z", "shot_mismatches")}}) + "\n")
+    bad = [r for r in plain + plain32 + served + at_load + checked if r["status"] != 0]
     if bad:
         raise RuntimeError(f"{len(bad)} requests failed: {bad[0].get('error')}")
     ls, lp, ll = S.latency_stats(served), S.latency_stats(plain), S.latency_stats(at_load)
-    rech = [r for r in served + at_load if r["rechecked"]]
+    lp32 = S.latency_stats(plain32)
+    rech = [r for r in served + at_load + checked if r["rechecked"]]
     dz = [abs(a["sum_z"] - b["sum_z"]) / max(1, a["n_qubits"]) for a, b in zip(served, plain)]
     r3 = lambda x: None if x is None else round(x, 3)
+    fid_bar = 1.0 - 1e-6
+    min_fid = min((r["fidelity"] for r in rech), default=None)
+    below = [r for r in rech if r["fidelity"] < fid_bar]
     return {
         "workload": f"{len(mix)} requests, Poisson arrivals at {args.serve_rate:g}/s, n log-uniform on "
                     f"[10, {args.serve_nmax}], 40% HEA (depth 4-16) / 30% GHZ+QFT / 30% U3+CZ (depth 8-24), "
@@ -230,13 +263,24 @@ def serving(q64, args):
                         "host_plan_ms": round(st["plan_ms"], 1)},
         "plain_fp64_loop": {"circuits_per_s": r3(lp["circuits_per_s"]), "p50_ms": r3(lp["p50_ms"]),
                             "p99_ms": r3(lp["p99_ms"]), "makespan_s": r3(lp["makespan_s"])},
+        "plain_fp32_loop": {"circuits_per_s": r3(lp32["circuits_per_s"]), "p50_ms": r3(lp32["p50_ms"]),
+                            "p99_ms": r3(lp32["p99_ms"]), "makespan_s": r3(lp32["makespan_s"])},
         "speedup_vs_plain": r3(ls["circuits_per_s"] / lp["circuits_per_s"]),
+        "speedup_vs_plain_fp32": r3(ls["circuits_per_s"] / lp32["circuits_per_s"]),
         "at_half_capacity": {"rate_per_s": r3(0.5 * cap), "circuits_per_s": r3(ll["circuits_per_s"]),
                              "p50_ms": r3(ll["p50_ms"]), "p99_ms": r3(ll["p99_ms"])},
-        "fp64_recheck": {"requests": len(rech),
-                         "min_fidelity": min((r["fidelity"] for r in rech), default=None),
+        "fp64_recheck": {"fidelity_definition": "|<a|b>|^2 / (<a|a> <b|b>), a = FP64 run, b = served FP32 state",
+                         "requests": len(rech), "requests_n_ge_25": sum(1 for r in rech if r["n_qubits"] >= 25),
+                         "fidelity_bar": fid_bar, "min_fidelity": min_fid,
+                         "bar_met": bool(rech) and not below, "requests_below_bar": len(below),
+                         "min_fidelity_raw": min((r["fidelity_raw"] for r in rech), default=None),
+                         "max_fidelity_raw": max((r["fidelity_raw"] for r in rech), default=None),
                          "max_abs_err": max((r["max_abs_err"] for r in rech), default=None),
-                         "sample_mismatches": int(sum(r["shot_mismatches"] for r in rech))},
+                         "max_norm_drift_fp32": max((abs(r["total_prob"] - 1.0) for r in rech), default=None),
+                         "max_sum_z_diff_per_qubit": max((abs(r["sum_z"] - r["sum_z_fp64"]) / r["n_qubits"]
+                                                          for r in rech), default=None),
+                         "sample_mismatches": int(sum(r["shot_mismatches"] for r in rech)),
+                         "samples_compared": int(1024 * len(rech))},
         "max_sum_z_diff_per_qubit_vs_fp64_loop": float(max(dz)) if dz else None,
         "qubit_hist": {int(k): int(v) for k, v in zip(*np.unique([d["n"] for d in mix], return_counts=True))},
     }
@@ -453,6 +497,7 @@ def main():
         out["paged_bv"] = paged_bv(q64, env, args.paged_bv - 1, int(args.paged_budget_gib * (1 << 30)))
     if args.precision_qubits:
         out["precision_form"] = precision_form(q64, args)
+    out["host"] = host_info()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_seconds)
     print(json.dumps(out), flush=True)

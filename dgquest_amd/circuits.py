@@ -108,6 +108,50 @@ def ghz_qft_closed_form(n: int):
     return (1.0 + np.exp(-2j * np.pi * rev / N)) / math.sqrt(2 * N)
 
 
+# The angle constant of the reference's GHZ_QFT testcases
+# (testcases/GHZ_QFT-30.c:6): pi mistyped as 3.1415265358979...
+PI_GHZ_QFT_TESTCASE = 3.141526535897932384626
+
+
+def ghz_qft_testcase(n: int, pi_: float = PI_GHZ_QFT_TESTCASE) -> List[Gate]:
+    """The circuit of the reference's GHZ_QFT-<n> programs
+    (testcases/GHZ_QFT-30.c:25-43): GHZ (H(0), CNOT chain), then H(0) and, for
+    i = 1 .. n-2, controlled-RZ(pi_ / 2^(i-j)) from each j < i onto i followed
+    by H(i); qubit n-1 is left as GHZ made it."""
+    c: List[Gate] = [("hadamard", (0,), ())]
+    for i in range(1, n):
+        c.append(("controlledNot", (i - 1, i), ()))
+    c.append(("hadamard", (0,), ()))
+    for i in range(1, n - 1):
+        for j in range(i):
+            c.append(("controlledRotateZ", (j, i), (pi_ * 0.5 ** (i - j),)))
+        c.append(("hadamard", (i,), ()))
+    return c
+
+
+def ghz_qft_testcase_amps(n: int, idx, pi_: float = PI_GHZ_QFT_TESTCASE):
+    """Amplitudes of ghz_qft_testcase(n) at basis indices idx, in closed form.
+
+    In the branch |b...b> of the GHZ state every target of a controlled RZ is
+    still the basis state |b> when the rotation acts, so each RZ only puts a
+    phase exp(i s theta / 2) (s = -1 for b = 0, +1 for b = 1) on the |1> part
+    of its control.  The branch ends as the product over j <= n-2 of
+    (|0> + (-1)^b exp(i phi_j) |1>) / sqrt(2), phi_j = s * sum_{i=j+1}^{n-2}
+    pi_ / 2^(i-j) / 2, times |b> on qubit n-1.  So a(y) = 2^{-n/2} * product
+    over the set bits j <= n-2 of y of (-1)^b exp(i phi_j), b = bit n-1 of y."""
+    import numpy as np
+
+    idx = np.asarray(idx, dtype=np.int64)
+    half = np.array([sum(pi_ * 0.5 ** (i - j) for i in range(j + 1, n - 1)) / 2.0 for j in range(n - 1)])
+    b = (idx >> (n - 1)) & 1
+    s = np.where(b == 1, 1.0, -1.0)
+    out = np.full(idx.shape, 2.0 ** (-n / 2), dtype=np.complex128)
+    for j in range(n - 1):
+        on = ((idx >> j) & 1) == 1
+        out = np.where(on, out * np.where(b == 1, -1.0, 1.0) * np.exp(1j * s * half[j]), out)
+    return out
+
+
 def layered_u3_cz(n: int, depth: int, seed: int) -> List[Gate]:
     """Per layer: U3(theta, phi, lambda) on every qubit (angles U[0, 2pi)), then
     CZ on a seeded perfect matching (shuffle, paired consecutively)."""

@@ -304,72 +304,6 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
     DQ_STORE_TILE(a, base);
 }
 
-// Occupancy target of the pipelined form: two register tiles live at once.
-template <typename T, int R>
-constexpr int pipe_waves_per_eu() {
-    constexpr int amp = (1 << R) * (int)(2 * sizeof(T) / 4);
-    constexpr int budget = 2 * amp + 40;
-    constexpr int w = 512 / budget;
-    return w > 8 ? 8 : (w < 1 ? 1 : w);
-}
-
-// Persistent, software-pipelined sweep of one state: workgroup g runs tiles
-// g, g + G, g + 2G, ... (G = gridDim.x) and issues the loads of its next tile
-// into a second register tile before it computes the current one, so each
-// wave keeps reads in flight while its VALU, LDS and scalar work runs.  The
-// interpreter's asm statement is a compiler barrier for memory operations,
-// which keeps the prefetch loads where they are issued.
-template <typename T, int R, int W, bool SPLIT>
-__global__ void __launch_bounds__(64 << W, (pipe_waves_per_eu<T, R>()))
-sweep_pipe_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
-                  const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
-                  const uint32_t *__restrict__ phase_aux, const uint32_t *__restrict__ block_aux) {
-    using GV = __attribute__((address_space(1))) typename V2<T>::type;
-    const dq_job job = jobs[blockIdx.y];
-    const dq_block *__restrict__ blk = blocks + job.block;
-    GV *__restrict__ st = (GV *)(job.state);
-    const uint32_t tid = threadIdx.x;
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t lane = tid & 63u;
-    const uint64_t msk = blk->nontile_mask;
-    const uint64_t ntiles = 1ull << __builtin_popcountll(msk);
-    const uint64_t G = gridDim.x;
-    const dq_phase *__restrict__ ph = phases + blk->phase_begin;
-    const uint32_t nph = blk->phase_count;
-    const uint32_t *__restrict__ pa = phase_aux + blk->phase_begin * DQ_PAUX;
-    const uint32_t *__restrict__ ba = block_aux + job.block * DQ_BAUX;
-    const uint64_t hi = job.base_hi;
-    const bool wave_perm = store_needs_barrier<W>(blk);
-    uint64_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    Regs<T, R> a, b;
-    uint64_t ta = tile_base_runs(ba, (uint32_t)t);
-    load_tile<T, R, W>(a, blk, st, ta, wv, lane);
-    bool lead = false;
-    // The prefetch loads are unconditional (a tile past the end is clamped to
-    // the last tile and its data left unused): a conditional load would make
-    // the compiler merge the two register tiles with copies, and a copy waits
-    // for the data, which turns the prefetch into a plain load.
-    while (true) {
-        const uint64_t t1 = t + G;
-        const uint64_t tb = tile_base_runs(ba, (uint32_t)(t1 < ntiles ? t1 : ntiles - 1));
-        load_tile<T, R, W>(b, blk, st, tb, wv, lane);
-        process_tile<T, R, W, SPLIT>(a, ph, nph, pa, prog, ta | hi, tid, lead);
-        if (wave_perm) __syncthreads();
-        DQ_STORE_TILE(a, ta);
-        lead = true;
-        if (t1 >= ntiles) break;
-        const uint64_t t2 = t1 + G;
-        ta = tile_base_runs(ba, (uint32_t)(t2 < ntiles ? t2 : ntiles - 1));
-        load_tile<T, R, W>(a, blk, st, ta, wv, lane);
-        process_tile<T, R, W, SPLIT>(b, ph, nph, pa, prog, tb | hi, tid, lead);
-        if (wave_perm) __syncthreads();
-        DQ_STORE_TILE(b, tb);
-        if (t2 >= ntiles) break;
-        t = t2;
-    }
-}
-
 // Small registers (n <= 13): whole state in LDS, one workgroup per job.  Ops
 // are in global-index form: tbit = target qubit, (c_wg, v_wg) = condition
 // over the full index.
@@ -1046,15 +980,11 @@ typedef void (*sweep_fn)(const dq_job *, const dq_block *, const dq_phase *, con
 template <typename T>
 struct SweepTable {
     sweep_fn fn[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
-    sweep_fn pipe[2][DQ_MAX_REG_BITS + 1][DQ_MAX_WAVE_BITS + 1];
     SweepTable() {
         memset(fn, 0, sizeof fn);
-        memset(pipe, 0, sizeof pipe);
 #define DQ_INST(R_, W_)                                                                  \
     fn[0][R_][W_] = sweep_kernel<T, R_, W_, false>;                                      \
-    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;                                       \
-    pipe[0][R_][W_] = sweep_pipe_kernel<T, R_, W_, false>;                               \
-    pipe[1][R_][W_] = sweep_pipe_kernel<T, R_, W_, true>;
+    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;
         DQ_INST(1, 0) DQ_INST(2, 0) DQ_INST(3, 0) DQ_INST(4, 0) DQ_INST(5, 0)
         DQ_INST(2, 1) DQ_INST(3, 1) DQ_INST(4, 1) DQ_INST(5, 1)
         DQ_INST(2, 2) DQ_INST(3, 2) DQ_INST(4, 2) DQ_INST(5, 2)
@@ -1082,23 +1012,6 @@ static bool use_split(int prec, int T) {
     if (g_split_mode == 0) return false;
     if (g_split_mode == 1) return true;
     return prec == DQ_F32 || T >= 12;
-}
-
-// persistent pipelined sweeps for single-state launches (DGQ_PIPE=1); off by
-// default: with two register tiles live the allocator copies prefetched
-// values, and each copy waits for its load (measured slower on MI355X)
-static int g_pipe_mode = -1;
-static bool use_pipe() {
-    if (g_pipe_mode < 0) {
-        const char *e = getenv("DGQ_PIPE");
-        g_pipe_mode = e ? atoi(e) : 0;
-    }
-    return g_pipe_mode != 0;
-}
-
-static sweep_fn lookup_pipe(int prec, int R, int W, bool split) {
-    if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS) return nullptr;
-    return prec == DQ_F64 ? g_tab64.pipe[split][R][W] : g_tab32.pipe[split][R][W];
 }
 
 static sweep_fn lookup(int prec, int R, int W, bool split = false) {
@@ -1178,10 +1091,6 @@ extern "C" int dq_device_init(int device) {
                         HIPC(hipFuncSetAttribute((const void *)g_tab64.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
                     if (g_tab32.fn[sp][R][W])
                         HIPC(hipFuncSetAttribute((const void *)g_tab32.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-                    if (g_tab64.pipe[sp][R][W])
-                        HIPC(hipFuncSetAttribute((const void *)g_tab64.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-                    if (g_tab32.pipe[sp][R][W])
-                        HIPC(hipFuncSetAttribute((const void *)g_tab32.pipe[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
                 }
         {
             hipDeviceProp_t prop;
@@ -1892,21 +1801,9 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             const uint64_t tiles = 1ull << (n - b0.tile_bits);
             if (tiles > 0x7fffffffull || nj > 65535 || ((tiles << (6 + b0.wave_bits)) >> 32))
                 return fail(DQ_ERR_UNSUPPORTED, "grid too large (at most 2^32 threads per launch)");
-            // (R = 5 keeps the one-pass form: two 32-amplitude tiles do not fit the registers)
-            sweep_fn pf = (nj == 1 && b0.reg_bits <= 4 && use_pipe()) ? lookup_pipe(prec, b0.reg_bits, b0.wave_bits, split)
-                                                                     : nullptr;
-            int per_cu = 0;
-            if (pf) HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pf, 64 << b0.wave_bits, lds));
-            const uint64_t resident = (uint64_t)(per_cu > 0 ? per_cu : 0) * (uint64_t)g_num_cus;
-            if (pf && resident > 0 && tiles >= 2 * resident) {
-                // persistent: every resident workgroup walks its share of the tiles
-                hipLaunchKernelGGL(pf, dim3((unsigned)resident, 1), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_paux, d_baux);
-            } else {
-                sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
-                hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
-                                   d_jobs + j0, d_blocks, d_phases, d_prog, d_paux, d_baux);
-            }
+            sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
+            hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
+                               d_jobs + j0, d_blocks, d_phases, d_prog, d_paux, d_baux);
         }
         HIPC(hipGetLastError());
     }

@@ -53,4 +53,26 @@ inline uint64_t gate_target_bits(const Gate &g) { return gate_is_diagonal(g.type
 inline int ctz64(uint64_t x) { return __builtin_ctzll(x); }
 inline int popc64(uint64_t x) { return __builtin_popcountll(x); }
 
+// Why a gate record that did not come through the API cannot be planned, or
+// nullptr when it can (the shapes the API itself records):
+//   phase types 0..8: no target, their qubits in the control mask;
+//   OP_ROTATE_Z_FAKE: kernel-internal in the reference
+//     (src/cpp/QuEST_backend.h:102), never a recorded gate;
+//   every other type: exactly one target qubit, not among the controls;
+//   OP_COLLAPSE: outcome op[0] is 0 or 1.
+// The caller checks that both masks lie inside the register.
+inline const char *gate_record_error(int type, uint64_t tmask, uint64_t cmask, const double *op) {
+    if (type < OP_ONE || type > OP_COLLAPSE) return "unknown gate type";
+    if (type == OP_ROTATE_Z_FAKE) return "type 9 (ROTATE_Z_FAKE) is not a recordable gate";
+    if (type <= OP_PHASE_SHIFT) {
+        if (tmask) return "a phase gate has no target (its qubits go in the control mask)";
+        if (!cmask) return "a phase gate needs at least one qubit in its control mask";
+        return nullptr;
+    }
+    if (!tmask || (tmask & (tmask - 1))) return "the gate needs exactly one target qubit";
+    if (tmask & cmask) return "the target qubit is also a control";
+    if (type == OP_COLLAPSE && !(op[0] == 0.0 || op[0] == 1.0)) return "a collapse outcome must be 0 or 1";
+    return nullptr;
+}
+
 }  // namespace dgq

@@ -14,6 +14,7 @@
 #include <sys/time.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -248,7 +249,44 @@ static bool rotate_lanes() {
     return on;
 }
 
-static bool rotation_for(const QuregState *s) { return s->rotate >= 0 ? s->rotate != 0 : rotate_lanes(); }
+// Lane rotation of a register's plans: the caller's setting, else the
+// per-circuit search's pick, else the default.
+static bool same_shape(const TuneParams &a, const TuneParams &b) {
+    return a.k == b.k && a.R == b.R && a.W == b.W && a.max_ops == b.max_ops && a.fuse_diagonals == b.fuse_diagonals;
+}
+
+static bool same_gate(const Gate &a, const Gate &b) {
+    return a.type == b.type && a.tmask == b.tmask && a.cmask == b.cmask && !memcmp(a.op, b.op, sizeof a.op);
+}
+
+// Was the register's last plan made from exactly what it now holds?
+static bool plan_is_cached(const QuregState *s, const TuneParams &t, bool rot) {
+    const auto &c = s->cache;
+    if (!c.valid || c.rotate != rot || !same_shape(c.tune, t) || c.init_kind != s->init_kind ||
+        c.init_index != s->init_index || c.init_value != s->init_value || c.gates.size() != s->pending.size())
+        return false;
+    for (size_t i = 0; i < c.gates.size(); i++)
+        if (!same_gate(c.gates[i], s->pending[i])) return false;
+    return true;
+}
+
+static void cache_plan(QuregState *s, const TuneParams &t, bool rot, const Plan &P) {
+    auto &c = s->cache;
+    c.valid = true;
+    c.gates = s->pending;
+    c.init_kind = s->init_kind;
+    c.init_index = s->init_index;
+    c.init_value = s->init_value;
+    c.tune = t;
+    c.rotate = rot;
+    c.plan = P;
+}
+
+static bool rotation_allowed(const QuregState *s) { return s->rotate >= 0 ? s->rotate != 0 : rotate_lanes(); }
+static bool rotation_for(const QuregState *s) {
+    if (s->rotate < 0 && s->rotate_pick >= 0) return s->rotate_pick != 0;
+    return rotation_allowed(s);
+}
 
 void flush(QuregState *s) {
     if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
@@ -295,8 +333,13 @@ void flush(QuregState *s) {
         s->last_dense_flops = D.flops;
     } else {
         Plan P;
-        plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P, -1,
-                     rotation_for(s));
+        const bool rot = rotation_for(s);
+        if (plan_is_cached(s, s->tune, rot)) {
+            P = s->cache.plan;
+        } else {
+            plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, s->tune, P, -1, rot);
+            cache_plan(s, s->tune, rot, P);
+        }
         run_plan(s, P);
         if (!P.moved.empty()) {  // the plan moved qubits: logical q now sits at moved[l2p[q]]
             for (int q = 0; q < s->n; q++) s->l2p[q] = P.moved[s->l2p[q]];
@@ -420,6 +463,42 @@ Qureg createQureg(int numQubits, QuESTEnv env) {
     return q;
 }
 
+// With DGQ_REPORT_FILE set, a register appends one JSON line about its final
+// state to that file when it is destroyed: size, precision, total
+// probability, P(qubit q = 1) for every logical qubit (1 - P(0), as
+// calcProbOfOutcome), and the first (up to 16) amplitudes in logical index
+// order, all at full precision.  It lets an unmodified QuEST program (which
+// prints a few values to 6 decimals, or nothing) be checked afterwards.
+static void report_register(QuregState *s, const char *path) {
+    ensure_probs(s);
+    const int n = s->n;
+    const uint64_t na = std::min<uint64_t>(16, 1ull << n);
+    std::vector<double> amps(2 * na);
+    if (s->paged) {
+        paged_read(s, 0, na, amps.data());
+    } else {
+        ensure_ws(s, 4096);
+        ENG(dq_read_amps(s->d_state, n, s->prec, s->l2p.data(), 0, na, amps.data(), s->d_ws, s->ws_cap, s->stream));
+    }
+    FILE *f = fopen(path, "a");
+    if (!f) {
+        fprintf(stderr, "dgquest: cannot open DGQ_REPORT_FILE %s\n", path);
+        return;
+    }
+    char comm[64] = "";
+    if (FILE *c = fopen("/proc/self/comm", "r")) {
+        if (fgets(comm, sizeof comm, c)) comm[strcspn(comm, "\n\"\\")] = 0;
+        fclose(c);
+    }
+    fprintf(f, "{\"program\": \"%s\", \"n\": %d, \"prec\": %d, \"paged\": %s, \"total_prob\": %.17g, \"p1\": [", comm, n,
+            s->prec, s->paged ? "true" : "false", s->p0[n]);
+    for (int q = 0; q < n; q++) fprintf(f, "%s%.17g", q ? ", " : "", 1.0 - s->p0[s->l2p[q]]);
+    fprintf(f, "], \"amps\": [");
+    for (uint64_t i = 0; i < na; i++) fprintf(f, "%s[%.17g, %.17g]", i ? ", " : "", amps[2 * i], amps[2 * i + 1]);
+    fprintf(f, "]}\n");
+    fclose(f);
+}
+
 void destroyQureg(Qureg qureg, QuESTEnv env) {
     (void)env;
     QuregState *s = st(qureg);
@@ -428,6 +507,8 @@ void destroyQureg(Qureg qureg, QuESTEnv env) {
         delete s;
         return;
     }
+    if (const char *path = getenv("DGQ_REPORT_FILE"))
+        if (*path) report_register(s, path);
     dq_stream_sync(s->stream);
     if (s->paged) paged_destroy(s->paged);
     else dq_free(s->d_state);
@@ -437,11 +518,54 @@ void destroyQureg(Qureg qureg, QuESTEnv env) {
 }
 
 void searchQuregTaskPartitions(Qureg qureg, QuESTEnv env) {
-    // The reference picks a task partition K for the recorded circuit
-    // (src/cpp/QuEST_backend_distributed.cpp:86-121).  Here the state is
-    // resident; the fusion shape comes from DGQ_TUNE or the defaults.
-    (void)qureg;
+    // The reference predicts, for the recorded circuit, the time of every task
+    // partition K with its sampled model and keeps the fastest
+    // (src/cpp/QuEST_backend_distributed.cpp:86-121,
+    // src/cpp/time_predictor.cpp:343-395).  With the state resident on one GPU
+    // the choice is the fused-sweep shape instead: the recorded gates are
+    // planned with every candidate shape (k, R, W), lanes rotating and fixed,
+    // each plan's device time is predicted from the measured per-shape cost
+    // model (tune.hpp), and the least is kept.  The register keeps that shape
+    // for later flushes; the next flush runs the winning plan without planning
+    // again.  DGQ_TUNE (an explicit shape) skips the search; so does a
+    // register with lane rotation switched off, for the rotation choice.  A
+    // recording register (dgq_create_recorder) is searched the same way, on
+    // the host only.
     (void)env;
+    QuregState *s = st(qureg);
+    if (!s || s->paged || s->dense_k || s->pending.empty() || getenv("DGQ_TUNE")) return;
+    // lane rotation: the caller's setting if made, else both when allowed
+    int rots[2] = {1, 0}, nrot = 2;
+    if (s->rotate >= 0 || !rotate_lanes()) {
+        rots[0] = rotation_allowed(s) ? 1 : 0;
+        nrot = 1;
+    }
+    double best = 0.0;
+    bool have = false;
+    Plan pick;
+    for (TuneParams c : tune_candidates(s->prec)) {
+        c.max_ops = s->tune.max_ops;
+        c.fuse_diagonals = s->tune.fuse_diagonals;
+        const TuneParams f = fit_shape(c, s->n, s->prec);
+        if (f.k != c.k || f.R != c.R || f.W != c.W) continue;  // the tile does not fit this register
+        const CostModel cm = tune_cost(s->prec, f);
+        for (int i = 0; i < nrot; i++) {
+            Plan P;
+            plan_circuit(s->pending, s->n, s->prec, s->init_kind, s->init_index, s->init_value, f, P, -1, rots[i] != 0);
+            const double ms = predict_plan_ms(P, s->prec, cm);
+            if (!have || ms < best) {
+                have = true;
+                best = ms;
+                s->tune = f;
+                s->rotate_pick = rots[i];
+                pick = std::move(P);
+            }
+        }
+    }
+    if (have) {
+        cache_plan(s, s->tune, rotation_for(s), pick);
+        s->predicted_ms = best;
+    }
 }
 
 int getNumQubits(Qureg qureg) { return (int)qureg.numQubitsRepresented; }
@@ -839,7 +963,19 @@ int dgq_set_lane_rotation(Qureg qureg, int on) {
     if (!s) return -1;
     if (!s->recorder) flush(s);  // a recording register only plans: nothing to flush
     s->rotate = on != 0;
+    s->rotate_pick = -1;
     return 0;
+}
+
+int dgq_last_search(Qureg qureg, int shape[4], double *predicted_ms) {
+    QuregState *s = st(qureg);
+    if (!s || !shape || !predicted_ms) return -1;
+    shape[0] = s->tune.k;
+    shape[1] = s->tune.R;
+    shape[2] = s->tune.W;
+    shape[3] = rotation_for(s) ? 1 : 0;
+    *predicted_ms = s->predicted_ms;
+    return s->predicted_ms > 0.0 ? 0 : 1;
 }
 
 int dgq_set_fusion(Qureg qureg, int on) {
@@ -919,7 +1055,8 @@ int dgq_plan_only(int n, int prec, const dgq_gate_rec *gates, int ngates, int k,
     std::vector<Gate> gl(ngates);
     const uint64_t full = (1ull << n) - 1;
     for (int i = 0; i < ngates; i++) {
-        if (gates[i].type < 0 || gates[i].type > OP_COLLAPSE || (gates[i].tmask & ~full) || (gates[i].cmask & ~full))
+        if ((gates[i].tmask & ~full) || (gates[i].cmask & ~full) ||
+            gate_record_error(gates[i].type, gates[i].tmask, gates[i].cmask, gates[i].op))
             return -1;
         gl[i].type = gates[i].type;
         gl[i].tmask = gates[i].tmask;

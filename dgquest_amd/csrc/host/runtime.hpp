@@ -44,8 +44,24 @@ struct QuregState {
     bool recorder = false;            // records and plans only, no device state
     int dense_k = 0;                  // > 0: flushes run as dense sweeps of this width (FP32)
     int rotate = -1;                  // lane rotation in plans: 1 on, 0 off, -1 the default (on; DGQ_ROTATE)
+    int rotate_pick = -1;             // searchQuregTaskPartitions' choice, used while rotate is -1
     double last_dense_flops = 0.0;    // device GEMM flops of the last dense flush
     dgq::PagedStore *paged = nullptr; // state in pinned host pieces, paged through the device
+    // The last plan made for this register and what it was made from.  A
+    // flush whose pending gates, initialisation, shape and lane rotation equal
+    // these runs the cached plan without planning again: a circuit recorded
+    // anew after an init* call, or the plan searchQuregTaskPartitions picked.
+    struct {
+        bool valid = false;
+        std::vector<dgq::Gate> gates;
+        int init_kind = 0;
+        uint64_t init_index = 0;
+        double init_value = 0.0;
+        dgq::TuneParams tune;
+        bool rotate = false;
+        dgq::Plan plan;
+    } cache;
+    double predicted_ms = 0.0;        // searchQuregTaskPartitions' predicted device ms (0: no search)
 };
 
 namespace dgq {

@@ -246,8 +246,9 @@ int make_req(const dgs_request &q, int prec, double recheck_fraction, Req &r) {
     r.gates.resize(q.n_gates);
     for (uint32_t i = 0; i < q.n_gates; i++) {
         const dgq_gate_rec &g = q.gates[i];
-        if (g.type < 0 || g.type > OP_COLLAPSE) return fail(DQ_ERR_ARG, "request: gate %u has type %d", i, g.type);
         if ((g.tmask | g.cmask) & ~full) return fail(DQ_ERR_ARG, "request: gate %u acts outside the register", i);
+        if (const char *why = gate_record_error(g.type, g.tmask, g.cmask, g.op))
+            return fail(DQ_ERR_ARG, "request: gate %u (type %d): %s", i, g.type, why);
         Gate &d = r.gates[i];
         d.type = g.type;
         d.tmask = g.tmask;
@@ -329,14 +330,18 @@ void collect(const Item &it, size_t i, Req &r) {
     bool ident = true;
     for (int q = 0; q < n; q++) ident &= r.l2p[q] == q;
     r.p0.resize(n + 1);
+    // P(q = 0) of the normalised state: the FP32 norm drifts off 1 by up to
+    // ~1e-6, which would otherwise bias every <Z_q> by as much
+    const double tot = p0[n];
+    const double inv = tot > 0.0 ? 1.0 / tot : 0.0;
     double sz = 0.0;
     for (int q = 0; q < n; q++) {
-        r.p0[q] = p0[r.l2p[q]];
+        r.p0[q] = p0[r.l2p[q]] * inv;
         sz += 2.0 * r.p0[q] - 1.0;
     }
-    r.p0[n] = p0[n];
+    r.p0[n] = tot;
     r.res.sum_z = sz;
-    r.res.total_prob = p0[n];
+    r.res.total_prob = tot;
     r.samples.resize(it.shots);
     for (uint32_t s = 0; s < it.shots; s++) r.samples[s] = ident ? sm[s] : to_logical(sm[s], r.l2p);
     r.res.batch_size = (int)it.reqs.size();
@@ -344,12 +349,18 @@ void collect(const Item &it, size_t i, Req &r) {
         const double *q64 = (const double *)(it.host + it.p064_off);
         const uint64_t *s64 = (const uint64_t *)(it.host + it.samp64_off);
         const double *cmp = (const double *)(it.host + it.cmp_off);
+        const double inv64 = q64[n] > 0.0 ? 1.0 / q64[n] : 0.0;
         double sz64 = 0.0;
-        for (int q = 0; q < n; q++) sz64 += 2.0 * q64[r.l2p[q]] - 1.0;
+        for (int q = 0; q < n; q++) sz64 += 2.0 * q64[r.l2p[q]] * inv64 - 1.0;
         uint32_t mis = 0;
         for (uint32_t s = 0; s < it.shots; s++) mis += s64[s] != sm[s];
         r.res.rechecked = 1;
-        r.res.fidelity = cmp[0] * cmp[0] + cmp[1] * cmp[1];
+        // cmp = re <a|b>, im <a|b>, <a|a>, <b|b>, max |a_i - b_i> (a: FP64, b: served)
+        const double ov = cmp[0] * cmp[0] + cmp[1] * cmp[1];
+        const double nn = cmp[2] * cmp[3];
+        r.res.fidelity_raw = ov;
+        r.res.norm_fp64 = cmp[2];
+        r.res.fidelity = nn > 0.0 ? ov / nn : 0.0;
         r.res.max_abs_err = cmp[4];
         r.res.sum_z_fp64 = sz64;
         r.res.shot_mismatches = mis;
@@ -656,8 +667,11 @@ private:
     }
 
     // Admission (lock held): the next launch group for this lane, or nullptr.
+    // Requests that cannot be batched (too large, or rechecked in FP64) run
+    // one per group on large lanes; with no large lanes the batch lanes take
+    // them too, so every request is admitted by some lane.
     Item *take(Lane &L) {
-        if (!L.batch) {
+        if (!L.batch || cfg_.large_lanes == 0) {
             for (auto &kv : planned_) {
                 Req *r = kv.second;
                 if (batchable(r)) continue;
@@ -818,8 +832,6 @@ int check_config(const dgs_config &c) {
     if (c.planner_threads < 1 || c.planner_threads > 64) return fail(DQ_ERR_ARG, "config: planner_threads 1..64");
     if (c.batch_lanes < 0 || c.large_lanes < 0 || c.batch_lanes + c.large_lanes < 1 || c.batch_lanes + c.large_lanes > 16)
         return fail(DQ_ERR_ARG, "config: 1..16 lanes");
-    if (c.large_lanes == 0 && c.batch_max_qubits < 40)
-        return fail(DQ_ERR_ARG, "config: without large lanes every register must be batchable");
     if (c.lane_depth < 1 || c.lane_depth > 8) return fail(DQ_ERR_ARG, "config: lane_depth 1..8");
     if (c.batch_max < 1 || c.batch_max > 4096) return fail(DQ_ERR_ARG, "config: batch_max 1..4096");
     if (!(c.recheck_fraction >= 0.0 && c.recheck_fraction <= 1.0)) return fail(DQ_ERR_ARG, "config: recheck_fraction 0..1");

@@ -40,7 +40,8 @@ class Result(C.Structure):
                 ("total_prob", C.c_double), ("t_arrive_ms", C.c_double), ("t_start_ms", C.c_double),
                 ("t_done_ms", C.c_double), ("rechecked", C.c_int), ("batch_size", C.c_int),
                 ("fidelity", C.c_double), ("max_abs_err", C.c_double), ("sum_z_fp64", C.c_double),
-                ("shot_mismatches", C.c_uint32), ("pad", C.c_uint32)]
+                ("shot_mismatches", C.c_uint32), ("pad", C.c_uint32), ("fidelity_raw", C.c_double),
+                ("norm_fp64", C.c_double)]
 
 
 def _bind(lib):

@@ -67,22 +67,28 @@ typedef struct dgs_request {
 #define DGS_RECHECK 1u
 #define DGS_NO_RECHECK 2u
 
+/* Probabilities are those of the normalised final state: P(q = 0) is the
+ * reduction's sum over amplitudes with bit q = 0 divided by the state's total
+ * probability (which rounding in FP32 moves off 1 by up to ~1e-6). */
 typedef struct dgs_result {
     uint64_t id;
     int status;                /* 0, or a DQ_ERR_* code */
     int n_qubits;
     double sum_z;              /* sum over qubits of 2 P(q = 0) - 1 */
-    double total_prob;
+    double total_prob;         /* sum_i |a_i|^2 of the served state, before normalising */
     double t_arrive_ms;        /* times since the server started */
     double t_start_ms;         /* launched on the device */
     double t_done_ms;          /* results back on the host */
     int rechecked;
     int batch_size;            /* registers in the launch that carried the request */
-    double fidelity;           /* |<psi_fp64|psi>|^2 when rechecked, else 0 */
+    double fidelity;           /* when rechecked: |<a|b>|^2 / (<a|a> <b|b>), a = the FP64 run's
+                                  state, b = the served state (1 = same state); else 0 */
     double max_abs_err;        /* max_i |a_i(fp64) - a_i| when rechecked */
     double sum_z_fp64;         /* sum_z of the FP64 run when rechecked */
     uint32_t shot_mismatches;  /* samples that differ between the two runs */
     uint32_t pad;
+    double fidelity_raw;       /* when rechecked: |<a|b>|^2 without the norms */
+    double norm_fp64;          /* when rechecked: <a|a> of the FP64 run */
 } dgs_result;
 
 void dgs_config_default(dgs_config *cfg);

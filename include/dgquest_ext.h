@@ -30,6 +30,13 @@ int dgq_set_tuning(Qureg qureg, int k, int R, int W, int max_ops);
  * positions keep their qubits, as in the reference's blocks. */
 int dgq_set_lane_rotation(Qureg qureg, int on);
 
+/* The shape searchQuregTaskPartitions chose for this register's recorded
+ * circuit: shape = k, R, W, lane rotation (1/0); *predicted_ms = the
+ * predicted device time of its plan.  Returns 0 after a search, 1 when the
+ * register's shape did not come from one (shape then holds its current
+ * setting). */
+int dgq_last_search(Qureg qureg, int shape[4], double *predicted_ms);
+
 /* Diagonal fusion on (1, default) or off (0): with it, uncontrolled
  * single-qubit gates run as real 2x2 rotations and their diagonal parts are
  * deferred and merged (fewer floating-point operations per gate). */

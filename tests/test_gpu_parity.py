@@ -321,3 +321,62 @@ def test_random_22q_default_plan(quest, env, oracle, tune):
         assert quest.measure(q, t) == oracle.measure(ref, t)[0]
     assert np.abs(quest.state_vector(q) - ref).max() <= TOL
     quest.destroyQureg(q, env)
+
+
+def _inverse_layered(circ):
+    inv = []
+    for name, qs, p in reversed(circ):
+        if name == "u3Gate":
+            th, ph, la = p
+            inv.append(("u3Gate", qs, (-th, -la, -ph)))
+        else:
+            inv.append((name, qs, p))
+    return inv
+
+
+# The 30-qubit benchmark's own plan: fusion width 7 on a 13-qubit tile of
+# 4 register and 3 wave bits, one block per sweep (max_ops large), lane
+# rotation on (the register default).
+BENCH_TUNE = (7, 4, 3, 100000)
+
+
+def test_30q_bench_plan_norm_and_inverse(quest, env):
+    """The benchmark circuit itself (30 qubits, depth 24, seed 2) with the
+    benchmark's plan: total probability 1 within 1e-12, then the circuit's
+    inverse, planned the same way, returns |0...0> within 1e-12."""
+    n = 30
+    circ = circuits.layered_u3_cz(n, 24, 2)
+    q = quest.createQureg(n, env)
+    quest.dgq_set_tuning(q, *BENCH_TUNE)
+    quest.dgq_set_lane_rotation(q, 1)
+    quest.apply_circuit(q, circ)
+    tp = quest.calcTotalProb(q)
+    st = quest.plan_stats(q)
+    print(f"30q bench plan: total prob 1{tp - 1:+.2e} stats={st}", flush=True)
+    assert abs(tp - 1.0) <= 1e-12
+    quest.apply_circuit(q, _inverse_layered(circ))
+    a0 = quest.getAmp(q, 0)
+    print(f"after inverse: a0 = {a0}", flush=True)
+    assert abs(a0.real - 1.0) <= 1e-12 and abs(a0.imag) <= 1e-12
+    for t in (0, 17, 29):
+        assert abs(quest.calcProbOfOutcome(q, t, 0) - 1.0) <= 1e-12
+    quest.destroyQureg(q, env)
+
+
+def test_26q_bench_plan_vs_oracle(quest, env, oracle):
+    """The benchmark's circuit family and plan (depth 24, 13-qubit tile, k=7,
+    lane rotation) at 26 qubits, where 13 qubits index tiles: every amplitude
+    within 1e-12 of the oracle, read through the register's qubit map."""
+    n = 26
+    circ = circuits.layered_u3_cz(n, 24, 2)
+    q = quest.createQureg(n, env)
+    quest.dgq_set_tuning(q, *BENCH_TUNE)
+    quest.dgq_set_lane_rotation(q, 1)
+    quest.apply_circuit(q, circ)
+    st = quest.plan_stats(q)
+    sv = quest.state_vector(q)
+    ref = run_oracle(oracle, n, circ)
+    err = float(np.abs(sv - ref).max())
+    print(f"26q bench plan: err={err:.3e} stats={st}", flush=True)
+    assert err <= TOL
+    quest.destroyQureg(q, env)

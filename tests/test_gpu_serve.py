@@ -208,3 +208,59 @@ def test_plain_loop_matches_oracle(quest, oracle):
         ref = oracle_state(oracle, d["n"], d["circuit"])
         assert np.abs(r["p0"] - p0_of(ref, d["n"])).max() <= 1e-12
         check_exact_samples(r["samples"], ref, S.uniforms(quest, d["shot_seed"], 1024))
+
+
+def test_serve_fp32_rechecked_large(quest):
+    """FP32 serving of 24-25-qubit requests, every one rechecked in FP64 on
+    the device: fidelity |<a|b>|^2 / (<a|a><b|b>) >= 1 - 1e-6 (a = FP64 run,
+    b = served state), the raw overlap consistent with it and the two norms,
+    and for GHZ + QFT sum <Z> against the closed form (FP64 run within 1e-10,
+    served FP32 within 1e-6 per qubit: P(q=0) comes from the normalised
+    state)."""
+    mix = [dict(kind="ghz_qft", n=25, circuit=CC.ghz_qft(25), shot_seed=1),
+           dict(kind="ghz_qft", n=24, circuit=CC.ghz_qft(24), shot_seed=2),
+           dict(kind="hea", n=24, circuit=CC.hea(24, 8, 3), shot_seed=3),
+           dict(kind="u3_cz", n=25, circuit=CC.layered_u3_cz(25, 12, 4), shot_seed=4)]
+    rs = make_requests(quest, mix, shots=256, flags=[S.DGS_RECHECK] * len(mix))
+    with S.Server(quest, prec=S.DQ_F32, hbm_budget=8 << 30) as srv:
+        out = srv.replay(rs)
+    for d, r in zip(mix, out):
+        n = d["n"]
+        print(f"{d['kind']} n={n} fid=1-{1 - r['fidelity']:.3e} raw=1{r['fidelity_raw'] - 1:+.3e} "
+              f"tot32=1{r['total_prob'] - 1:+.3e} tot64=1{r['norm_fp64'] - 1:+.3e} "
+              f"maxerr={r['max_abs_err']:.2e} dz={abs(r['sum_z'] - r['sum_z_fp64']):.2e}", flush=True)
+        assert r["status"] == 0 and r["rechecked"] == 1
+        assert r["fidelity"] >= 1 - 1e-6
+        assert abs(r["norm_fp64"] - 1.0) <= 1e-12
+        assert abs(r["fidelity_raw"] - r["fidelity"] * r["norm_fp64"] * r["total_prob"]) <= 1e-12
+        assert r["max_abs_err"] <= 1e-5
+        assert abs(r["sum_z"] - r["sum_z_fp64"]) <= 1e-6 * n
+        if d["kind"] == "ghz_qft":
+            p = np.abs(CC.ghz_qft_closed_form(n)) ** 2
+            idx = np.arange(len(p))
+            sz = sum(2 * p[((idx >> q) & 1) == 0].sum() - 1 for q in range(n))
+            del p, idx
+            assert abs(r["sum_z_fp64"] - sz) <= 1e-10
+            assert abs(r["sum_z"] - sz) <= 1e-6 * n
+
+
+def test_serve_without_large_lanes(quest, oracle):
+    """Batch lanes only: registers above batch_max_qubits and requests
+    rechecked in FP64 (neither batchable) still run, one per launch group,
+    and every request completes."""
+    mix = small_mix(6, 26, 12)
+    mix.append(dict(kind="hea", n=17, depth=4, circuit_seed=9, shot_seed=9, arrive_s=0.0, circuit=CC.hea(17, 4, 9)))
+    flags = [0, S.DGS_RECHECK, 0, 0, S.DGS_RECHECK, 0, 0]
+    rs = make_requests(quest, mix, shots=64, flags=flags)
+    with S.Server(quest, prec=S.DQ_F32, hbm_budget=1 << 30, batch_lanes=1, large_lanes=0,
+                  batch_max_qubits=14) as srv:
+        out = srv.replay(rs)
+        st = srv.stats()
+    print(st, flush=True)
+    assert st["rechecked"] == 2
+    for d, r, f in zip(mix, out, flags):
+        assert r["status"] == 0
+        assert r["rechecked"] == (1 if f == S.DGS_RECHECK else 0)
+        ref = oracle_state(oracle, d["n"], d["circuit"])
+        sz = (2 * p0_of(ref, d["n"])[:-1] - 1).sum()
+        assert abs(r["sum_z"] - sz) <= 2e-5 * d["n"]

@@ -115,6 +115,51 @@ def test_planner_rejects_bad_records():
     assert L.dgq_plan_only(10, 3, r, 0, 6, 4, 2, 100, st) == -1
 
 
+@pytest.mark.parametrize("case", [
+    (9, 1 << 2, 0, 0.0),           # ROTATE_Z_FAKE: kernel-internal, never recorded
+    (9, 0, 1 << 2, 0.0),
+    (4, 1 << 2, 1 << 3, 0.0),      # phase gate with a target
+    (2, 0, 0, 0.0),                # phase gate on no qubit
+    (8, 1 << 1, 0, 0.5),
+    (14, 0, 0, 0.0),               # Hadamard without a target
+    (11, 0b11, 0, 0.0),            # two target bits
+    (18, 1 << 3, 1 << 3, 0.0),     # target among the controls
+    (10, 0, 1 << 2, 0.3),          # Rz with no target
+    (10, 0b101, 0, 0.3),
+    (19, 1 << 4, 0, 2.0),          # collapse outcome not 0/1
+    (19, 0, 0, 1.0),
+    (20, 1 << 4, 0, 0.0),          # unknown type
+    (-1, 1 << 4, 0, 0.0),
+])
+def test_planner_rejects_malformed_records(case):
+    """Records from outside the API (dgq_plan_only here, server requests
+    through the same check) are refused unless they have the shape the API
+    records for their type."""
+    L = C.CDLL(os.path.join(LIB, "libQuEST_f64.so"))
+    typ, tm, cm, op0 = case
+    r = (Rec * 1)()
+    r[0].type, r[0].tmask, r[0].cmask = typ, tm, cm
+    r[0].op[0] = op0
+    r[0].op[1] = 1.0
+    st = (C.c_int * 8)()
+    assert L.dgq_plan_only(10, 2, r, 1, 6, 4, 2, 100, st) == -1
+
+
+@pytest.mark.parametrize("case", [
+    (4, 0, 1 << 2, 0.0), (8, 0, 0b110, 0.4), (0, 0, 1, 0.0), (10, 1 << 2, 0, 0.3), (10, 1 << 2, 1 << 5, 0.3),
+    (18, 1 << 3, 0b110000, 0.0), (19, 1 << 4, 0, 1.0), (19, 1 << 4, 0, 0.0), (14, 1 << 9, 0, 0.0),
+])
+def test_planner_accepts_api_shaped_records(case):
+    L = C.CDLL(os.path.join(LIB, "libQuEST_f64.so"))
+    typ, tm, cm, op0 = case
+    r = (Rec * 1)()
+    r[0].type, r[0].tmask, r[0].cmask = typ, tm, cm
+    r[0].op[0] = op0
+    r[0].op[1] = 1.0
+    st = (C.c_int * 8)()
+    assert L.dgq_plan_only(10, 2, r, 1, 6, 4, 2, 100, st) == 0
+
+
 def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)

@@ -228,3 +228,24 @@ def test_grid_generator_no_repeat():
             q = g[1][0]
             assert last.get(q) != g[0]
             last[q] = g[0]
+
+
+@pytest.mark.parametrize("n", [3, 6, 9, 11])
+def test_ghz_qft_testcase_closed_form(oracle, n):
+    """The closed form used to check the reference's GHZ_QFT-30 program run
+    through the drop-in: pinned against the oracle (every amplitude) and the
+    dense brute force at small n; P(q = 1) = 1/2 for every qubit."""
+    c = circuits.ghz_qft_testcase(n)
+    s = np.zeros(1 << n, dtype=np.complex128)
+    s[0] = 1.0
+    v = oracle.run(s, c)
+    cf = circuits.ghz_qft_testcase_amps(n, np.arange(1 << n))
+    assert np.abs(v - cf).max() <= 1e-13
+    if n <= 9:
+        assert np.abs(H.dense_run(n, c) - cf).max() <= 1e-12
+    p = np.abs(cf) ** 2
+    idx = np.arange(1 << n)
+    for q in range(n):
+        assert abs(p[((idx >> q) & 1) == 1].sum() - 0.5) <= 1e-13
+    # the mistyped pi matters: with the true pi the amplitudes differ
+    assert np.abs(circuits.ghz_qft_testcase_amps(n, idx, math.pi) - cf).max() > 1e-6

@@ -111,8 +111,7 @@ def test_hea_circuit():
 def test_config_validation(q64):
     lib = S._bind(q64.lib)
     for bad in [dict(prec=3), dict(hbm_budget=1000), dict(planner_threads=0), dict(batch_lanes=0, large_lanes=0),
-                dict(lane_depth=0), dict(batch_max=0), dict(recheck_fraction=1.5),
-                dict(large_lanes=0, batch_max_qubits=22)]:
+                dict(lane_depth=0), dict(batch_max=0), dict(recheck_fraction=1.5)]:
         c = S.default_config(q64, **bad)
         assert not lib.dgs_create(C.byref(c)), bad
         assert b"config" in lib.dgs_last_error()
