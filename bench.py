@@ -30,7 +30,8 @@ recheck) against the plain one-at-a-time FP64 loop on the same arrivals:
 circuits/s of both (arrivals at 2000/s, above either loop's capacity, so both
 are measured work-bound), their ratio, p50/p99 latency, and latency at half
 the served capacity.  paged_bv: a Bernstein-Vazirani register forced into pinned
-host pieces by a small device budget.
+host pieces by a small device budget.  paged_layered: a layered circuit paged
+the same way, whose sweeps give the copies something to overlap with.
 """
 import argparse
 import json
@@ -63,6 +64,8 @@ def parse():
     ap.add_argument("--paged-bv", type=int, default=28,
                     help="qubits of the host-paged Bernstein-Vazirani run (0 = skip)")
     ap.add_argument("--paged-budget-gib", type=float, default=1.0)
+    ap.add_argument("--paged-layered", type=str, default="30,6,4",
+                    help="n,depth,budget_GiB of the host-paged layered U3+CZ run ('' = skip)")
     ap.add_argument("--serve", type=int, default=400, help="requests of the serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the throughput run, requests/s (above capacity: both loops "
@@ -141,6 +144,84 @@ def cpu_baseline(circ, n, seconds):
     }
 
 
+def paged_register(q64, env, n, budget_bytes):
+    """An n-qubit FP64 register forced into pinned host pieces by a device
+    budget of budget_bytes (DGQ_HBM_BUDGET is read when a register is created)."""
+    old = os.environ.get("DGQ_HBM_BUDGET")
+    os.environ["DGQ_HBM_BUDGET"] = str(int(budget_bytes))
+    try:
+        return q64.createQureg(n, env)
+    finally:
+        if old is None:
+            del os.environ["DGQ_HBM_BUDGET"]
+        else:
+            os.environ["DGQ_HBM_BUDGET"] = old
+
+
+def paged_stats(q64, q):
+    """(piece qubits, log2 pieces, stats dict) of a paged register's last flush."""
+    import ctypes as C
+
+    L = q64.lib
+    L.dgq_paged_info.restype = C.c_int
+    L.dgq_paged_info.argtypes = [type(q), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    nl, K = C.c_int(), C.c_int()
+    st = (C.c_double * 10)()
+    if L.dgq_paged_info(q, C.byref(nl), C.byref(K), st) != 0:
+        raise RuntimeError("register is not paged")
+    keys = ["sections", "remaps", "pieces_run", "pieces_skipped", "copy_ms", "hidden_ms", "compute_ms", "wall_ms",
+            "bytes_in", "bytes_out"]
+    return nl.value, K.value, dict(zip(keys, list(st)))
+
+
+def overlap_figures(st):
+    """Copy/compute overlap of a paged flush from its stream timeline: copy
+    time that ran while a sweep ran, over the shorter of the two totals (1 =
+    the shorter one hidden entirely), and over all copy time."""
+    copy, comp, hid = st["copy_ms"], st["compute_ms"], st["hidden_ms"]
+    short = min(copy, comp)
+    return {"copy_ms": round(copy, 1), "compute_ms": round(comp, 1), "hidden_copy_ms": round(hid, 1),
+            "overlap_of_shorter": round(hid / short, 4) if short > 0 else None,
+            "hidden_fraction_of_copy": round(hid / copy, 4) if copy > 0 else None,
+            "link_GBps": round((st["bytes_in"] + st["bytes_out"]) / (st["wall_ms"] * 1e-3) / 1e9, 1)
+            if st["wall_ms"] > 0 else None}
+
+
+def paged_layered(q64, env, n, depth, budget_bytes, seed=2):
+    """A compute-carrying paged flush: the layered U3+CZ circuit on n qubits
+    in pinned host pieces, paged through a device budget of budget_bytes.
+    Every section runs fused sweeps on the pieces while the next pieces copy
+    in and the previous ones copy out.  Checked against the same circuit on a
+    resident register: P(q = 0) for every qubit and the total probability."""
+    from dgquest_amd import circuits
+
+    circ = circuits.layered_u3_cz(n, depth, seed)
+    q = paged_register(q64, env, n, budget_bytes)
+    t0 = time.perf_counter()
+    q64.apply_circuit(q, circ)
+    q64.dgq_flush(q)
+    flush_s = time.perf_counter() - t0
+    nl, K, st = paged_stats(q64, q)
+    p_paged = [q64.calcProbOfOutcome(q, i, 0) for i in range(n)]
+    tot_paged = q64.calcTotalProb(q)
+    q64.destroyQureg(q, env)
+    r = q64.createQureg(n, env)
+    q64.apply_circuit(r, circ)
+    p_res = [q64.calcProbOfOutcome(r, i, 0) for i in range(n)]
+    q64.destroyQureg(r, env)
+    out = {
+        "workload": f"{n}-qubit layered U3+CZ circuit, depth {depth}, seed {seed}, FP64 ({16 << n >> 30} GiB in "
+                    f"pinned host memory), device budget {budget_bytes >> 30} GiB",
+        "pieces": 1 << K, "piece_qubits": nl, "gates": len(circ),
+        "flush_s": round(flush_s, 3), "sections": int(st["sections"]), "qubit_moves": int(st["remaps"]),
+        "pieces_run": int(st["pieces_run"]),
+        "max_p0_diff_vs_resident": float(max(abs(a - b) for a, b in zip(p_paged, p_res))),
+        "total_prob": tot_paged,
+    }
+    out.update(overlap_figures(st))
+    return out
+
+
 def paged_bv(q64, env, n_data, budget_bytes, seed=3):
     """Config-3 shape: Bernstein-Vazirani on n_data + 1 qubits held in pinned
     host pieces and paged through a device budget of `budget_bytes`.  Returns
@@ -150,28 +231,15 @@ def paged_bv(q64, env, n_data, budget_bytes, seed=3):
     from dgquest_amd import circuits
 
     circ, hidden = circuits.bernstein_vazirani(n_data, seed)
-    old = os.environ.get("DGQ_HBM_BUDGET")
-    os.environ["DGQ_HBM_BUDGET"] = str(int(budget_bytes))
-    try:
-        q = q64.createQureg(n_data + 1, env)
-    finally:
-        if old is None:
-            del os.environ["DGQ_HBM_BUDGET"]
-        else:
-            os.environ["DGQ_HBM_BUDGET"] = old
-    L = q64.lib
-    L.dgq_paged_info.restype = C.c_int
-    L.dgq_paged_info.argtypes = [type(q), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
-    nl, K = C.c_int(), C.c_int()
-    st = (C.c_double * 10)()
-    if L.dgq_paged_info(q, C.byref(nl), C.byref(K), st) != 0:
-        raise RuntimeError("register is not paged")
+    t_alloc = time.perf_counter()
+    q = paged_register(q64, env, n_data + 1, budget_bytes)
+    alloc_s = time.perf_counter() - t_alloc
+    paged_stats(q64, q)
     t0 = time.perf_counter()
     q64.apply_circuit(q, circ)
     q64.dgq_flush(q)
     flush_s = time.perf_counter() - t0
-    L.dgq_paged_info(q, C.byref(nl), C.byref(K), st)
-    stats = list(st)
+    nl, K, st = paged_stats(q64, q)
     q64.seed([seed])
     got = 0
     t1 = time.perf_counter()
@@ -180,19 +248,17 @@ def paged_bv(q64, env, n_data, budget_bytes, seed=3):
             got |= 1 << i
     measure_s = time.perf_counter() - t1
     q64.destroyQureg(q, env)
-    sections, remaps, run, skipped, copy_ms, hidden_ms, comp_ms, wall_ms, b_in, b_out = stats
-    return {
+    out = {
         "workload": f"{n_data + 1}-qubit Bernstein-Vazirani (seed {seed}), FP64 "
                     f"({16 << (n_data + 1) >> 30} GiB in pinned host memory), device budget {budget_bytes >> 30} GiB",
-        "pieces": 1 << K.value, "piece_qubits": nl.value,
+        "pieces": 1 << K, "piece_qubits": nl,
         "recovered_hidden_string": got == hidden,
-        "flush_s": round(flush_s, 3), "measure_all_s": round(measure_s, 3),
-        "sections": int(sections), "qubit_moves": int(remaps),
-        "pieces_run": int(run), "pieces_skipped_zero": int(skipped),
-        "copy_ms": round(copy_ms, 1), "compute_ms": round(comp_ms, 1),
-        "overlap_fraction": round(hidden_ms / copy_ms, 4) if copy_ms > 0 else None,
-        "link_GBps": round((b_in + b_out) / (wall_ms * 1e-3) / 1e9, 1) if wall_ms > 0 else None,
+        "pinned_alloc_s": round(alloc_s, 3), "flush_s": round(flush_s, 3), "measure_all_s": round(measure_s, 3),
+        "sections": int(st["sections"]), "qubit_moves": int(st["remaps"]),
+        "pieces_run": int(st["pieces_run"]), "pieces_skipped_zero": int(st["pieces_skipped"]),
     }
+    out.update(overlap_figures(st))
+    return out
 
 
 def serving(q64, args):
@@ -495,6 +561,9 @@ def main():
         out["serving"] = serving(q64, args)
     if args.paged_bv:
         out["paged_bv"] = paged_bv(q64, env, args.paged_bv - 1, int(args.paged_budget_gib * (1 << 30)))
+    if args.paged_layered:
+        pn, pd, pb = (float(x) for x in args.paged_layered.split(","))
+        out["paged_layered"] = paged_layered(q64, env, int(pn), int(pd), int(pb * (1 << 30)))
     if args.precision_qubits:
         out["precision_form"] = precision_form(q64, args)
     out["host"] = host_info()

@@ -86,13 +86,29 @@ struct Regs {
 #define DQ_RUNS 10
 
 // Byte offset of a thread's part of an exchange address: xor of the parts of
-// its set thread bits.
+// its set thread bits.  The wave bits are wave-uniform (wv is a scalar), so
+// their part is formed with scalar selects; each lane bit costs a bit extract
+// (0 or all ones), an and and an xor, with no branch.
 template <int W>
-__device__ __forceinline__ uint32_t thread_part(const uint32_t *__restrict__ t, uint32_t tid) {
+__device__ __forceinline__ uint32_t thread_part(const uint32_t *__restrict__ t, uint32_t lane, uint32_t wv) {
+    uint32_t tv[6 + W];
+#pragma unroll
+    for (int i = 0; i < 6 + W; i++) tv[i] = t[i];
     uint32_t x = 0;
 #pragma unroll
-    for (int i = 0; i < 6 + W; i++) x ^= ((tid >> i) & 1u) ? t[i] : 0u;
+    for (int w = 0; w < W; w++) x ^= ((wv >> w) & 1u) ? tv[6 + w] : 0u;
+#pragma unroll
+    for (int i = 0; i < 6; i++) x ^= (uint32_t)(-(int32_t)((lane >> i) & 1u)) & tv[i];
     return x;
+}
+
+// An LDS element at byte offset `off` of the workgroup's LDS.  The sweep
+// kernels declare no static LDS (the engine checks their static group
+// segment is 0 when it initialises a device), so their dynamic LDS starts at
+// address 0 and an offset is the address itself: no base add per access.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) T *lds_at(uint32_t off) {
+    return (__attribute__((address_space(3))) T *)(size_t)off;
 }
 
 // Re-distribute the tile from the layout of the previous phase to that of
@@ -106,36 +122,35 @@ __device__ __forceinline__ uint32_t thread_part(const uint32_t *__restrict__ t, 
 // per register index j, both formed from the host's tables (ex = this phase's
 // phase_aux), one vector xor per access.
 template <int R, int W, bool SPLIT, typename T>
-__device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restrict__ ex, bool lead_barrier, uint32_t tid) {
+__device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restrict__ ex, bool lead_barrier,
+                                         uint32_t lane, uint32_t wv) {
     constexpr int NR = 1 << R;
     const uint32_t *__restrict__ xa = ex + DQ_XA;
     const uint32_t *__restrict__ xb = ex + DQ_XB;
-    const uint32_t sa = thread_part<W>(ex + DQ_TA, tid);
-    const uint32_t sb = thread_part<W>(ex + DQ_TB, tid);
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    auto at = [](uint32_t off) { return reinterpret_cast<T *>(lds_raw + off); };
+    const uint32_t sa = thread_part<W>(ex + DQ_TA, lane, wv);
+    const uint32_t sb = thread_part<W>(ex + DQ_TB, lane, wv);
     if (!SPLIT) {
         if (lead_barrier) __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            T *q = at(sa ^ xa[j]);
+            auto *q = lds_at<T>(sa ^ xa[j]);
             q[0] = a.re[j];
             q[1] = a.im[j];
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
-            const T *q = at(sb ^ xb[j]);
+            const auto *q = lds_at<T>(sb ^ xb[j]);
             a.re[j] = q[0];
             a.im[j] = q[1];
         }
     } else {
         if (lead_barrier) __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *at(sa ^ xa[j]) = a.re[j];
+        for (int j = 0; j < NR; j++) *lds_at<T>(sa ^ xa[j]) = a.re[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.re[j] = *at(sb ^ xb[j]);
+        for (int j = 0; j < NR; j++) a.re[j] = *lds_at<T>(sb ^ xb[j]);
         // for R <= 4 the imaginary pass forms its 2^(R+1) addresses again (one
         // xor each) instead of keeping them in registers across the passes; at
         // R = 5 recomputing them spills more than keeping them.  FP64 tiles of
@@ -145,10 +160,10 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restri
         if constexpr (R <= 4 && !(std::is_same<T, double>::value && W == 3)) asm volatile("" : "+v"(sa2), "+v"(sb2));
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *at(sa2 ^ xa[j]) = a.im[j];
+        for (int j = 0; j < NR; j++) *lds_at<T>(sa2 ^ xa[j]) = a.im[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) a.im[j] = *at(sb2 ^ xb[j]);
+        for (int j = 0; j < NR; j++) a.im[j] = *lds_at<T>(sb2 ^ xb[j]);
     }
 }
 
@@ -191,6 +206,15 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t msk, uint64_t t) {
     return base;
 }
 
+// Element `lane` of the 64 consecutive amplitudes starting at the
+// wave-uniform address p: a 32-bit lane byte offset on a scalar base, so the
+// access is one global load/store with a scalar address and no 64-bit vector
+// address arithmetic.
+template <typename P>
+__device__ __forceinline__ P lane_ptr(P p, uint32_t lane) {
+    return (P)((const __attribute__((address_space(1))) char *)p + lane * (uint32_t)sizeof(*p));
+}
+
 // Load a tile into registers: lane l = qubit l, register / wave bits =
 // ld_reg / ld_wave.  The wave index wv is wave-uniform (a scalar), so every
 // address is "scalar base + lane * sizeof(V)" with no vector math.
@@ -214,7 +238,7 @@ __device__ __forceinline__ void load_tile(Regs<T, R> &a, const dq_block *__restr
 #pragma unroll
             for (int r = 0; r < R; r++)
                 if ((j >> r) & 1) u |= qo[r];
-            const V v = __builtin_nontemporal_load(st + u + lane);
+            const V v = __builtin_nontemporal_load(lane_ptr(st + u, lane));
             a.re[j] = v.x;
             a.im[j] = v.y;
         }
@@ -248,11 +272,10 @@ __device__ __forceinline__ void load_tile(Regs<T, R> &a, const dq_block *__restr
         _Pragma("unroll") for (int j = 0; j < (1 << R); j++) {                            \
             uint64_t u = (BASE) | wb_;                                                    \
             _Pragma("unroll") for (int r = 0; r < R; r++) if ((j >> r) & 1) u |= qo_[r];  \
-            GV *p_ = st + u;                                                              \
             typename V2<T>::type v_;                                                      \
             v_.x = (A).re[j];                                                             \
             v_.y = (A).im[j];                                                             \
-            __builtin_nontemporal_store(v_, p_ + lane);                                   \
+            __builtin_nontemporal_store(v_, lane_ptr(st + u, lane));                      \
         }                                                                                 \
     } while (0)
 
@@ -276,9 +299,9 @@ __device__ __forceinline__ bool store_needs_barrier(const dq_block *__restrict__
 template <typename T, int R, int W, bool SPLIT>
 __device__ __forceinline__ void process_tile(Regs<T, R> &a, const dq_phase *__restrict__ ph, uint32_t nph,
                                              const uint32_t *__restrict__ pa, const unsigned char *__restrict__ prog,
-                                             uint64_t cbase, uint32_t tid, bool lead) {
+                                             uint64_t cbase, uint32_t tid, uint32_t wv, bool lead) {
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W, SPLIT, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid);
+        if (p > 0) exchange<R, W, SPLIT, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid & 63u, wv);
         AsmOps<T, R>::run(a, prog + pa[p * DQ_PAUX], cbase, tid);
     }
 }
@@ -299,7 +322,7 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
     Regs<T, R> a;
     load_tile<T, R, W>(a, blk, st, base, wv, lane);
     process_tile<T, R, W, SPLIT>(a, phases + blk->phase_begin, blk->phase_count, phase_aux + blk->phase_begin * DQ_PAUX,
-                                 prog, base | job.base_hi, tid, false);
+                                 prog, base | job.base_hi, tid, wv, false);
     if (store_needs_barrier<W>(blk)) __syncthreads();
     DQ_STORE_TILE(a, base);
 }
@@ -884,14 +907,26 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     for (int b = 6; b < 9; b++)
         if ((tid >> b) & 1u) tg |= hq[b];
     const uint32_t tz = dswz(tid);
-    for (uint32_t i = 0; i < NT / DENSE_THREADS; i++) {
-        uint64_t g = tg;
-        for (uint32_t b = 9; b < T; b++)
-            if ((i >> (b - 9)) & 1u) g |= hq[b];
-        const typename V2<float>::type v = __builtin_nontemporal_load(st + g);
-        const uint32_t z = tz ^ dswz(i * DENSE_THREADS);
-        re[z] = v.x;
-        im[z] = v.y;
+    // i < NI = 2^(T - 9) (2..16) runs over the tile bits 9 and up, whose qubits
+    // are scalars.  Every thread issues 16 loads before using the first;
+    // below T = 13 index i stands for element i mod NI, so the extra loads,
+    // LDS writes and stores repeat an element with the same value (no branch).
+    const uint32_t NI = NT / DENSE_THREADS;
+    const uint64_t h9 = T > 9 ? hq[9] : 0, h10 = T > 10 ? hq[10] : 0, h11 = T > 11 ? hq[11] : 0,
+                   h12 = T > 12 ? hq[12] : 0;
+    auto gaddr = [&](uint32_t i) {
+        return tg | ((i & 1u) ? h9 : 0ull) | ((i & 2u) ? h10 : 0ull) | ((i & 4u) ? h11 : 0ull) | ((i & 8u) ? h12 : 0ull);
+    };
+    {
+        typename V2<float>::type v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = __builtin_nontemporal_load(st + gaddr((uint32_t)i & (NI - 1)));
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t z = tz ^ dswz(((uint32_t)i & (NI - 1)) * DENSE_THREADS);
+            re[z] = v[i].x;
+            im[z] = v[i].y;
+        }
     }
     const size_t msz = (size_t)(2u << KL) * (2u << KL);
     for (uint32_t o = 0; o < blk->op_count; o++) {
@@ -899,15 +934,16 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
         dense_op<KL>(re, im, T, op, mats + op.mat * msz, ctab + (size_t)(blk->op_begin + o) * DENSE_COLS, tid);
     }
     __syncthreads();
-    for (uint32_t i = 0; i < NT / DENSE_THREADS; i++) {
-        uint64_t g = tg;
-        for (uint32_t b = 9; b < T; b++)
-            if ((i >> (b - 9)) & 1u) g |= hq[b];
-        const uint32_t z = tz ^ dswz(i * DENSE_THREADS);
-        typename V2<float>::type v;
-        v.x = re[z];
-        v.y = im[z];
-        __builtin_nontemporal_store(v, st + g);
+    {
+        typename V2<float>::type v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t z = tz ^ dswz(((uint32_t)i & (NI - 1)) * DENSE_THREADS);
+            v[i].x = re[z];
+            v[i].y = im[z];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) __builtin_nontemporal_store(v[i], st + gaddr((uint32_t)i & (NI - 1)));
     }
 }
 
@@ -1087,10 +1123,16 @@ extern "C" int dq_device_init(int device) {
         for (int sp = 0; sp < 2; sp++)
             for (int R = 0; R <= DQ_MAX_REG_BITS; R++)
                 for (int W = 0; W <= DQ_MAX_WAVE_BITS; W++) {
-                    if (g_tab64.fn[sp][R][W])
-                        HIPC(hipFuncSetAttribute((const void *)g_tab64.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
-                    if (g_tab32.fn[sp][R][W])
-                        HIPC(hipFuncSetAttribute((const void *)g_tab32.fn[sp][R][W], hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                    for (const void *fn : {(const void *)g_tab64.fn[sp][R][W], (const void *)g_tab32.fn[sp][R][W]}) {
+                        if (!fn) continue;
+                        HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+                        // the exchange addresses LDS by offset from 0 (lds_at): no static LDS allowed
+                        hipFuncAttributes fa;
+                        HIPC(hipFuncGetAttributes(&fa, fn));
+                        if (fa.sharedSizeBytes != 0)
+                            return fail(DQ_ERR_UNSUPPORTED, "sweep kernel (R=%d, W=%d) has %zu bytes of static LDS", R, W,
+                                        (size_t)fa.sharedSizeBytes);
+                    }
                 }
         {
             hipDeviceProp_t prop;
@@ -1174,11 +1216,14 @@ extern "C" int dq_stream_wait_event(void *stream, void *ev) {
 static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Upper bound of the device program built from n_ops descriptors in n_phases
-// phases: at most 128 bytes per descriptor (a complex matrix in FP64 behind a
-// workgroup-condition op; a register table of 2^R entries spans 1 + 2^R / 4
-// descriptors and at most 16 + 16 * 2^R bytes), an END op per phase, and
-// padding so the interpreter's fixed-size header loads stay in the buffer.
-static size_t prog_bound(uint32_t n_phases, uint32_t n_ops) { return (size_t)n_ops * 128 + (size_t)n_phases * 16 + 256; }
+// phases: at most 320 bytes per descriptor.  A complex matrix in FP64 behind
+// a workgroup-condition op takes 128.  A register table of 2^R entries spans
+// 1 + 2^R / 4 descriptors; fused with shears and emitted once per assignment
+// of at most two folded workgroup bits, it takes at most 4 x (32 + 16 +
+// 16 * 2^R + 16 * R) bytes (2560 at R = 5, against 9 x 320).  Plus an END op
+// per phase, and padding so the interpreter's fixed-size header loads stay in
+// the buffer.
+static size_t prog_bound(uint32_t n_phases, uint32_t n_ops) { return (size_t)n_ops * 320 + (size_t)n_phases * 16 + 256; }
 
 extern "C" size_t dq_apply_ws_bytes(uint32_t n_blocks, uint32_t n_phases, uint32_t n_ops, uint32_t n_jobs) {
     return align_up(n_blocks * sizeof(dq_block), 256) + align_up(n_phases * sizeof(dq_phase), 256) +
@@ -1423,35 +1468,84 @@ struct ProgBuilder {
     // condition in a run of diagonal ops that holds a table is multiplied into
     // that run's table; the table is emitted before the next non-diagonal op,
     // fused with it when that op is a run of unconditioned shears.
+    // NEG / PHASE ops of such a run whose only condition is one workgroup bit
+    // (controlled-Z or controlled phases with a qubit outside the tile) are
+    // folded too, for up to kWgFoldBits distinct bits: the table is emitted
+    // once per assignment of those bits, each copy guarded by its assignment
+    // and holding the factors that apply there.  A workgroup runs exactly one
+    // copy and skips the others, so the conditioned factors cost no vector
+    // instruction.
+    static constexpr int kWgFoldBits = 2;  // the program size bound (prog_bound) assumes at most 2
+    struct WgDiag {
+        uint64_t c_wg, v_wg;
+        uint32_t c_reg, v_reg;
+        double fr, fi;
+    };
     void build(const dq_op *ops, uint32_t count, uint32_t &n_out) {
         n_out = 0;
         std::vector<unsigned char> op;
         std::vector<double> tr(NR, 1.0), ti(NR, 0.0);
         bool pending = false;  // tr/ti hold a table not yet emitted
+        std::vector<WgDiag> wgd;  // workgroup-conditioned factors of the pending table
+        uint64_t wg_bits = 0;     // their workgroup bits
         auto is_diag = [](const dq_op &o) {
             return o.kind == DQ_OP_NEG || o.kind == DQ_OP_PHASE || o.kind == DQ_OP_DIAGR;
         };
-        auto mul = [&](uint32_t c, uint32_t v, double fr, double fi) {
+        auto mul_into = [&](std::vector<double> &xr, std::vector<double> &xi, uint32_t c, uint32_t v, double fr,
+                            double fi) {
             for (int j = 0; j < NR; j++)
                 if (((uint32_t)j & c) == v) {
-                    const double r = tr[j] * fr - ti[j] * fi, i = tr[j] * fi + ti[j] * fr;
-                    tr[j] = r;
-                    ti[j] = i;
+                    const double r = xr[j] * fr - xi[j] * fi, i = xr[j] * fi + xi[j] * fr;
+                    xr[j] = r;
+                    xi[j] = i;
                 }
         };
+        auto mul = [&](uint32_t c, uint32_t v, double fr, double fi) { mul_into(tr, ti, c, v, fr, fi); };
         auto flush = [&](const std::vector<unsigned char> *slots, uint32_t shear_mask) {
-            op.clear();
-            const bool any = pending && encode_table(op, tr, ti, slots, shear_mask);
+            if (pending && wg_bits) {
+                const int nb = __builtin_popcountll(wg_bits);
+                for (uint32_t a = 0; a < (1u << nb); a++) {
+                    uint64_t val = 0, m = wg_bits;  // assignment a deposited on the bits
+                    for (int i = 0; m; i++, m &= m - 1)
+                        if ((a >> i) & 1) val |= m & (~m + 1);
+                    std::vector<double> vr = tr, vi = ti;
+                    for (const WgDiag &d : wgd)
+                        if ((val & d.c_wg) == d.v_wg) mul_into(vr, vi, d.c_reg, d.v_reg, d.fr, d.fi);
+                    op.clear();
+                    if (!encode_table(op, vr, vi, slots, shear_mask)) {
+                        if (!slots) continue;  // nothing to do for these workgroups
+                        op.clear();
+                        encode(op, code("SHEARS"), shear_mask, 0, 0, slots->data(), slots->size());
+                    }
+                    emit(op, wg_bits, val);
+                    n_out++;
+                }
+            } else {
+                op.clear();
+                const bool any = pending && encode_table(op, tr, ti, slots, shear_mask);
+                if (any || slots) {
+                    if (!any) {
+                        op.clear();
+                        encode(op, code("SHEARS"), shear_mask, 0, 0, slots->data(), slots->size());
+                    }
+                    emit(op, 0, 0);
+                    n_out++;
+                }
+            }
             std::fill(tr.begin(), tr.end(), 1.0);
             std::fill(ti.begin(), ti.end(), 0.0);
             pending = false;
-            if (!any) {
-                if (!slots) return;
-                op.clear();
-                encode(op, code("SHEARS"), shear_mask, 0, 0, slots->data(), slots->size());
-            }
-            emit(op, 0, 0);
-            n_out++;
+            wgd.clear();
+            wg_bits = 0;
+        };
+        // a NEG / PHASE with one workgroup-bit condition, folded into the pending table
+        auto fold_wg = [&](const dq_op &o, double fr, double fi) {
+            if (o.c_thr || __builtin_popcountll(o.c_wg) != 1) return false;
+            if (!(wg_bits & o.c_wg) && __builtin_popcountll(wg_bits) >= kWgFoldBits) return false;
+            wgd.push_back({o.c_wg, o.v_wg, o.c_reg, o.v_reg, fr, fi});
+            wg_bits |= o.c_wg;
+            pending = true;
+            return true;
         };
         bool fold_run = false;  // the current run of diagonal ops holds an unconditioned table
         bool in_run = false;
@@ -1518,6 +1612,7 @@ struct ProgBuilder {
                         pending = true;
                         continue;
                     }
+                    if (fold_run && o.c_wg && fold_wg(o, -1.0, 0.0)) continue;
                     encode(op, cond_variant("NEG_ALL", "NEG_B", "NEG_BB", "NEG_GEN", o.c_reg, o.v_reg),
                            jmask(o.c_reg, o.v_reg, -1), o.c_thr, o.v_thr, nullptr, 0);
                     break;
@@ -1527,6 +1622,7 @@ struct ProgBuilder {
                         pending = true;
                         continue;
                     }
+                    if (fold_run && o.c_wg && fold_wg(o, val(o, 0), val(o, 1))) continue;
                     encode(op, cond_variant("PHASE_ALL", "PHASE_B", "PHASE_BB", "PHASE_GEN", o.c_reg, o.v_reg),
                            jmask(o.c_reg, o.v_reg, -1), o.c_thr, o.v_thr, m, 2 * ts);
                     break;

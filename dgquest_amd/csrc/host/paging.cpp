@@ -415,6 +415,7 @@ void paged_flush(QuregState *s) {
     s->pending.clear();
     s->init_kind = DQ_INIT_LOAD;
     s->probs_dirty = true;
+    s->det_base = false;
     P.st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -518,6 +519,7 @@ void paged_write(QuregState *s, uint64_t start, uint64_t count, const double *re
         }
     }
     s->probs_dirty = true;
+    s->det_base = false;
 }
 
 }  // namespace dgq

@@ -291,6 +291,7 @@ static bool rotation_for(const QuregState *s) {
 void flush(QuregState *s) {
     if (s->recorder) engine_fatal("a recording register cannot execute or read its state", DQ_ERR_UNSUPPORTED);
     if (s->pending.empty() && s->init_kind == DQ_INIT_LOAD) return;
+    s->det_base = false;
     if (s->paged) {
         const size_t nrec = s->pending.size();
         paged_flush(s);
@@ -359,8 +360,11 @@ void flush(QuregState *s) {
 void ensure_probs(QuregState *s) {
     flush(s);
     if (!s->probs_dirty) return;
+    s->det_pending = 0;
+    s->det_slack = 0.0;
     if (s->paged) {
         paged_probs(s);
+        s->det_base = true;
         return;
     }
     ensure_ws(s, dq_prob_ws_bytes(s->n));
@@ -387,6 +391,7 @@ void materialize_mapping(QuregState *s) {
     for (const Plan &P : passes) run_plan(s, P);
     for (int q = 0; q < n; q++) s->p2l[s->l2p[q]] = q;
     s->probs_dirty = true;
+    s->det_base = false;
     if (!gates.empty()) {
         s->pending.insert(s->pending.end(), gates.begin(), gates.end());
         flush(s);
@@ -585,6 +590,7 @@ static void set_init(Qureg q, int kind, uint64_t idx, double val) {
     s->init_index = idx;
     s->init_value = val;
     s->probs_dirty = true;
+    s->det_base = false;
 }
 
 void initZeroState(Qureg qureg) { set_init(qureg, DQ_INIT_CLASSICAL, 0, 0.0); }
@@ -918,17 +924,41 @@ qreal collapseToOutcome(Qureg qureg, const int q, int outcome) {
     return prob;
 }
 
+// P(q = 0) of a paged register from its last P(0) pass when every gate
+// recorded since is the collapse of a deterministic measurement: each such
+// collapse drops a branch of probability at most its slack and rescales the
+// rest, which moves any P(q = 0) by at most 2 * slack in all.  Only a value
+// that decides the outcome rule the same way anywhere in that range is
+// taken (the paged P(0) pass reads every host piece over the host link).
+static bool paged_cached_zero_prob(QuregState *s, int q, double *zero_prob) {
+    if (!s->paged || !s->det_base || s->init_kind != DQ_INIT_LOAD) return false;
+    if (s->pending.size() != (size_t)s->det_pending) return false;
+    const double p = s->p0[s->l2p[q]], m = 2.0 * s->det_slack;
+    if (!(p + m < REAL_EPS || 1.0 - (p - m) < REAL_EPS)) return false;
+    *zero_prob = p;
+    return true;
+}
+
 int measureWithStats(Qureg qureg, int q, qreal *outcomeProb) {
     v_target(qureg, q, __func__);
-    const qreal zeroProb = calcProbOfOutcome(qureg, q, 0);
+    QuregState *s = st(qureg);
+    double cached = 0.0;
+    const qreal zeroProb = paged_cached_zero_prob(s, q, &cached) ? (qreal)cached : calcProbOfOutcome(qureg, q, 0);
     // src/QuEST_common.c:97-115
     int outcome;
+    const bool determined = zeroProb < REAL_EPS || 1 - zeroProb < REAL_EPS;
     if (zeroProb < REAL_EPS) outcome = 1;
     else if (1 - zeroProb < REAL_EPS) outcome = 0;
     else outcome = (mt_genrand_real1() > zeroProb);
     const qreal prob = (outcome == 0) ? zeroProb : 1 - zeroProb;
     if (outcomeProb) *outcomeProb = prob;
+    const bool chain = s->paged && s->det_base && s->init_kind == DQ_INIT_LOAD &&
+                       s->pending.size() == (size_t)s->det_pending;
     record_collapse(qureg, q, outcome, prob);
+    if (chain && determined) {
+        s->det_pending++;
+        s->det_slack += std::max(0.0, 1.0 - (double)prob);
+    }
     return outcome;
 }
 

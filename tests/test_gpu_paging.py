@@ -176,3 +176,41 @@ def test_paged_bernstein_vazirani(quest, env):
     st = paged_info(quest, q)
     print(f"nl={info['nl']} K={info['K']} stats={st}", flush=True)
     assert got == s
+
+
+def test_paged_deterministic_measurements_then_random_one(quest, env, oracle):
+    """Deterministic measurements of a paged register reuse one P(0) pass (their
+    collapses stay pending); the ancilla's 50/50 measurement after them must
+    execute the pending collapses first.  Outcomes, probabilities and the final
+    state equal the oracle's for the same seed."""
+    import time
+
+    n_data = 13
+    circ, s = circuits.bernstein_vazirani(n_data, 5)
+    n = n_data + 1
+    q, _ = paged_register(quest, env, n, 64 << 10)
+    quest.apply_circuit(q, circ)
+    ref = oracle_state(oracle, n, circ)
+    keys = [17, 4]
+    quest.seed(keys)
+    oracle.seed(keys)
+    got = 0
+    t0 = time.perf_counter()
+    for i in range(n_data):
+        o = quest.measure(q, i)
+        ro, rp = oracle.measure(ref, i)
+        assert o == ro
+        got |= o << i
+    t_det = time.perf_counter() - t0
+    assert got == s
+    import ctypes as C
+
+    p = C.c_double()
+    o = quest.measureWithStats(q, n_data, C.byref(p))
+    ro, rp = oracle.measure(ref, n_data)
+    assert o == ro
+    assert abs(p.value - rp) <= TOL and abs(rp - 0.5) <= TOL
+    print(f"13 deterministic measurements in {t_det * 1e3:.1f} ms", flush=True)
+    assert np.abs(quest.state_vector(q) - ref).max() <= TOL
+    assert abs(quest.calcTotalProb(q) - 1.0) <= TOL
+    quest.destroyQureg(q, env)
