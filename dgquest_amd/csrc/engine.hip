@@ -480,6 +480,17 @@ __global__ void __launch_bounds__(256) prob_stage2(const double *__restrict__ pa
     if (tid == 0) out[q] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Device memory of the allocation-free P(0) entry point
+// (dq_prob_zero_all_noalloc): per-workgroup partial sums of one chunk of at
+// most 2^NA_CHUNK_BITS amplitudes, then NA_CHUNK_BITS + 1 results per chunk
+// for up to NA_MAX_QUBITS qubits; 2.2 MiB on every device the library runs on.
+#define NA_CHUNK_BITS 30
+#define NA_MAX_QUBITS 36
+#define NA_PART_DOUBLES ((1u << (NA_CHUNK_BITS - PROB_WG_BITS)) * PROB_SLOTS)
+#define NA_CHUNK_DOUBLES ((1u << (NA_MAX_QUBITS - NA_CHUNK_BITS)) * (NA_CHUNK_BITS + 1))
+static_assert((size_t)NA_PART_DOUBLES * sizeof(double) % 256 == 0, "chunk results stay 256-byte aligned");
+__device__ __align__(256) double scratch[NA_PART_DOUBLES + NA_CHUNK_DOUBLES];
+
 // ----------------------------------------------------------------------------
 // read-out of many states of one size: P(0) per qubit, total probability and
 // samples drawn by inverse transform over a tree of probability sums
@@ -1076,6 +1087,10 @@ unsigned g_stage_used = 0;
 unsigned g_stage_next = 0;
 int g_device = -1;
 int g_num_cus = 256;
+// Address of the allocation-free read-out's `scratch` array on each device
+// (hipGetSymbolAddress at dq_device_init; null until then).
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {};
 }  // namespace
 
 static int stage_upload(void *d_dst, const std::vector<unsigned char> &src, void *stream) {
@@ -1117,7 +1132,15 @@ static int stage_upload(void *d_dst, const std::vector<unsigned char> &src, void
 // C ABI: plumbing
 // ----------------------------------------------------------------------------
 extern "C" int dq_device_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(DQ_ERR_ARG, "dq_device_init: device %d out of range", device);
     HIPC(hipSetDevice(device));
+    if (!g_scratch_dev[device]) {
+        // each device holds its own copy of `scratch`; host code uses this
+        // address, never the array's name (a host-side stand-in)
+        void *p = nullptr;
+        HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = (double *)p;
+    }
     if (g_device != device) {
         // allow the large LDS tiles (up to 128 KiB) on every sweep instantiation
         for (int sp = 0; sp < 2; sp++)
@@ -1943,6 +1966,56 @@ extern "C" int dq_prob_zero_all(const void *state, int n, int prec, double *out,
     if (rc) return rc;
     HIPC(hipMemcpyAsync(out, d_out, (n + 1) * sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream));
     HIPC(hipStreamSynchronize((hipStream_t)stream));
+    return DQ_OK;
+}
+
+// ---- P(0) of every qubit without allocation or workspace --------------------
+// For callers that may not allocate device memory (code under stream capture,
+// serving loops): the partial sums live in a file-scope device array.  A
+// state is reduced in chunks of at most 2^NA_CHUNK_BITS amplitudes, each by
+// prob_stage1/prob_stage2 (for n <= NA_CHUNK_BITS exactly the launches of
+// dq_prob_zero_all_async); prob_combine sums the chunks' results.  The
+// array `scratch` (defined with the P(0) kernels) holds the partial sums of
+// one chunk, then the results of every chunk.
+
+// out[q] for q < n and the total out[n] from nchunks chunk results of
+// NA_CHUNK_BITS + 1 doubles each (chunk c = amplitudes with index bits
+// NA_CHUNK_BITS.. equal to c); one thread per output, chunks summed in order.
+__global__ void __launch_bounds__(64) prob_combine(const double *__restrict__ chunks, uint32_t nchunks, int n,
+                                                   double *__restrict__ out) {
+    const int q = threadIdx.x;
+    if (q > n) return;
+    constexpr int S = NA_CHUNK_BITS + 1;
+    double acc = 0.0;
+    for (uint32_t c = 0; c < nchunks; c++) {
+        if (q < NA_CHUNK_BITS) acc += chunks[(uint64_t)c * S + q];
+        else if (q == n || !((c >> (q - NA_CHUNK_BITS)) & 1u)) acc += chunks[(uint64_t)c * S + NA_CHUNK_BITS];
+    }
+    out[q] = acc;
+}
+
+extern "C" int dq_prob_zero_all_noalloc(const void *state, int n, int prec, double *d_out, void *stream) {
+    if (!state || !d_out || n < 1 || (prec != DQ_F32 && prec != DQ_F64))
+        return fail(DQ_ERR_ARG, "dq_prob_zero_all_noalloc: bad arguments");
+    if (n > NA_MAX_QUBITS)
+        return fail(DQ_ERR_ARG, "dq_prob_zero_all_noalloc: %d qubits, above its limit of %d", n, NA_MAX_QUBITS);
+    int dev = -1;
+    HIPC(hipGetDevice(&dev));
+    double *scr = (dev >= 0 && dev < kMaxDevices) ? g_scratch_dev[dev] : nullptr;
+    if (!scr) return fail(DQ_ERR_ARG, "dq_prob_zero_all_noalloc: dq_device_init(%d) has not been called", dev);
+    if (n <= NA_CHUNK_BITS)
+        return dq_prob_zero_all_async(state, n, prec, d_out, scr, (size_t)NA_PART_DOUBLES * sizeof(double), stream);
+    double *chunk_res = scr + NA_PART_DOUBLES;
+    const uint32_t nchunks = 1u << (n - NA_CHUNK_BITS);
+    const size_t amp_bytes = prec == DQ_F64 ? 16 : 8;
+    for (uint32_t c = 0; c < nchunks; c++) {
+        const void *piece = (const unsigned char *)state + ((size_t)c << NA_CHUNK_BITS) * amp_bytes;
+        const int rc = dq_prob_zero_all_async(piece, NA_CHUNK_BITS, prec, chunk_res + (size_t)c * (NA_CHUNK_BITS + 1), scr,
+                                              (size_t)NA_PART_DOUBLES * sizeof(double), stream);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(prob_combine, dim3(1), dim3(64), 0, (hipStream_t)stream, chunk_res, nchunks, n, d_out);
+    HIPC(hipGetLastError());
     return DQ_OK;
 }
 

@@ -214,6 +214,19 @@ size_t dq_prob_ws_bytes(int n_qubits);
 int dq_prob_zero_all_async(const void *state, int n_qubits, int prec, double *d_out,
                            void *d_ws, size_t ws_bytes, void *stream);
 
+/* Same reduction with no workspace argument and no allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops).  The partial sums
+ * live in a device array of the library (2.2 MiB, one copy on every device
+ * it runs on, held until the process ends); dq_device_init must have been
+ * called for the current device.  n_qubits <= 36 (larger is refused with
+ * DQ_ERR_ARG).  Up to 30 qubits it runs exactly the launches of
+ * dq_prob_zero_all_async (bit-identical results); above, the state is
+ * reduced in chunks of 2^30 amplitudes whose results a last kernel sums.
+ * Asynchronous on `stream`.  Every call on a device shares that array, so
+ * calls that overlap on one device (other streams, other threads) race. */
+int dq_prob_zero_all_noalloc(const void *state, int n_qubits, int prec, double *d_out, void *stream);
+
 /* ---- dense sweeps: FP32 state, dense 2^k x 2^k blocks on MFMA --------------
  * A dense block loads a tile of T qubits (qubits 0..5 always tile bits
  * 0..5; tile bit b = qubit tile_q[b]) into LDS, applies its dense ops in
