@@ -312,6 +312,22 @@ def serving(q64, args):
         raise RuntimeError(f"{len(bad)} requests failed: {bad[0].get('error')}")
     ls, lp, ll = S.latency_stats(served), S.latency_stats(plain), S.latency_stats(at_load)
     lp32 = S.latency_stats(plain32)
+    # service time of each request in the one-at-a-time loops (always
+    # backlogged: start = the previous request's end), by register size
+    svc64 = [r["t_done_ms"] - r["t_start_ms"] for r in plain]
+    svc32 = [r["t_done_ms"] - r["t_start_ms"] for r in plain32]
+    nq = [d["n"] for d in mix]
+    big = 23  # registers above the server's batch limit (22 qubits) run one per launch group
+    buckets = {}
+    for lo, hi in ((10, 16), (17, 22), (23, 26), (27, 30)):
+        idx = [i for i in range(len(mix)) if lo <= nq[i] <= hi]
+        if idx:
+            buckets[f"{lo}-{hi}"] = {"requests": len(idx),
+                                     "fp64_ms_mean": round(sum(svc64[i] for i in idx) / len(idx), 3),
+                                     "fp32_ms_mean": round(sum(svc32[i] for i in idx) / len(idx), 3),
+                                     "fp64_ms_sum": round(sum(svc64[i] for i in idx), 1)}
+    t64 = sum(svc64)
+    t32_big = sum(svc32[i] for i in range(len(mix)) if nq[i] >= big)
     rech = [r for r in served + at_load + checked if r["rechecked"]]
     dz = [abs(a["sum_z"] - b["sum_z"]) / max(1, a["n_qubits"]) for a, b in zip(served, plain)]
     r3 = lambda x: None if x is None else round(x, 3)
@@ -331,6 +347,11 @@ def serving(q64, args):
                             "p99_ms": r3(lp["p99_ms"]), "makespan_s": r3(lp["makespan_s"])},
         "plain_fp32_loop": {"circuits_per_s": r3(lp32["circuits_per_s"]), "p50_ms": r3(lp32["p50_ms"]),
                             "p99_ms": r3(lp32["p99_ms"]), "makespan_s": r3(lp32["makespan_s"])},
+        "plain_service_ms_by_size": buckets,
+        # an upper bound of served / plain FP64: every register up to the
+        # batch limit free (perfect batching), every larger one at its FP32
+        # one-at-a-time service time
+        "bound_vs_plain_fp64": r3(t64 / t32_big) if t32_big > 0 else None,
         "speedup_vs_plain": r3(ls["circuits_per_s"] / lp["circuits_per_s"]),
         "speedup_vs_plain_fp32": r3(ls["circuits_per_s"] / lp32["circuits_per_s"]),
         "at_half_capacity": {"rate_per_s": r3(0.5 * cap), "circuits_per_s": r3(ll["circuits_per_s"]),
