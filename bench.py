@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--serve-nmax", type=int, default=30)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-dump", type=str, default="", help="write per-request results (JSON lines) here")
+    ap.add_argument("--serve-replay-only", action="store_true",
+                    help="run only the served replay of the serving mix (to trace its launch groups) and exit")
     ap.add_argument("--precision-qubits", type=int, default=32,
                     help="grid-circuit size of the FP32 MFMA precision-form run (0 = skip)")
     ap.add_argument("--precision-k", type=str, default="5,6,7")
@@ -373,6 +375,24 @@ def serving(q64, args):
     }
 
 
+def serve_replay_only(q64, args):
+    """The served replay of the serving mix alone (no plain loops, no recheck
+    pass): what a kernel trace of the server's launch groups needs."""
+    from dgquest_amd import circuits
+    from dgquest_amd import serve as S
+
+    mix = circuits.serving_mix(args.serve, seed=5, rate=args.serve_rate, n_max=args.serve_nmax, build=False)
+    recs = [S.record(q64, d["n"], circuits.mix_circuit(d)) for d in mix]
+    rs = S.RequestSet(recs, [1024] * len(mix), [d["shot_seed"] for d in mix])
+    with S.Server(q64, prec=S.DQ_F32, hbm_budget=int(args.serve_budget_gb * 1e9), recheck_fraction=0.0) as srv:
+        served = srv.replay(rs, [d["arrive_s"] for d in mix])
+        st = srv.stats()
+    ls = S.latency_stats(served)
+    return {"served_replay": {"requests": len(mix), "n_max": args.serve_nmax, "circuits_per_s": ls["circuits_per_s"],
+                              "p50_ms": ls["p50_ms"], "p99_ms": ls["p99_ms"], "launch_groups": int(st["batches"]),
+                              "batched_registers": int(st["batched_registers"])}}
+
+
 def precision_form(q64, args):
     """Config 4: a 4 x (n/4) grid circuit of depth 20 (sqrt X/Y/W per qubit per
     cycle, CZ on four edge patterns in rotation), seed 4.  FP64 butterflies on
@@ -450,6 +470,9 @@ def main():
     q64 = Q.load(2)
     eng = Engine()
     env = q64.createQuESTEnv()
+    if args.serve_replay_only:
+        print(json.dumps(serve_replay_only(q64, args)), flush=True)
+        return
     n = args.qubits
     circ = circuits.layered_u3_cz(n, args.depth, args.seed)
     q = q64.createQureg(n, env)

@@ -113,15 +113,15 @@ __device__ __forceinline__ __attribute__((address_space(3))) T *lds_at(uint32_t 
 
 // Re-distribute the tile from the layout of the previous phase to that of
 // this one through LDS (each thread writes its 2^R amplitudes, then reads
-// 2^R).  SPLIT moves the real parts, then the imaginary parts, through an LDS
-// array of scalars: half the LDS footprint (more workgroups per CU), two more
-// barriers.  An element's LDS slot is its tile index under a linear swizzle
+// 2^R).  Exchange modes (XM): 0 moves whole amplitudes; 1 moves the real
+// parts, then the imaginary parts, through an LDS array of scalars: half the
+// LDS footprint (more workgroups per CU), two more barriers.  An element's LDS slot is its tile index under a linear swizzle
 // the host picks per exchange (xor of the low bits with images of the high
 // bits) so that the lanes served together by the LDS in either layout hit
 // distinct banks; a byte offset is then the thread's part xor a uniform part
 // per register index j, both formed from the host's tables (ex = this phase's
 // phase_aux), one vector xor per access.
-template <int R, int W, bool SPLIT, typename T>
+template <int R, int W, int XM, typename T>
 __device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restrict__ ex, bool lead_barrier,
                                          uint32_t lane, uint32_t wv) {
     constexpr int NR = 1 << R;
@@ -129,7 +129,7 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restri
     const uint32_t *__restrict__ xb = ex + DQ_XB;
     const uint32_t sa = thread_part<W>(ex + DQ_TA, lane, wv);
     const uint32_t sb = thread_part<W>(ex + DQ_TB, lane, wv);
-    if (!SPLIT) {
+    if (XM == 0) {
         if (lead_barrier) __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
@@ -186,7 +186,7 @@ __device__ __forceinline__ uint64_t tile_base_runs(const uint32_t *__restrict__ 
 // Occupancy target (waves per SIMD) of a sweep instantiation: the register
 // tile takes 2^R amplitudes x (2 * sizeof(T) / 4) VGPRs; allow 40 more for
 // addresses, coefficients and temporaries.
-template <typename T, int R>
+template <typename T, int R, int XM>
 constexpr int sweep_waves_per_eu() {
     constexpr int amp = (1 << R) * (int)(2 * sizeof(T) / 4);
     constexpr int budget = amp + 40;
@@ -296,18 +296,18 @@ __device__ __forceinline__ bool store_needs_barrier(const dq_block *__restrict__
 // The phases of a block on one tile: exchanges through LDS between phases,
 // the op program of each phase in the assembly interpreter.  lead: a barrier
 // before the first exchange (LDS may still be read for a previous tile).
-template <typename T, int R, int W, bool SPLIT>
+template <typename T, int R, int W, int XM>
 __device__ __forceinline__ void process_tile(Regs<T, R> &a, const dq_phase *__restrict__ ph, uint32_t nph,
                                              const uint32_t *__restrict__ pa, const unsigned char *__restrict__ prog,
                                              uint64_t cbase, uint32_t tid, uint32_t wv, bool lead) {
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W, SPLIT, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid & 63u, wv);
+        if (p > 0) exchange<R, W, XM, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid & 63u, wv);
         AsmOps<T, R>::run(a, prog + pa[p * DQ_PAUX], cbase, tid);
     }
 }
 
-template <typename T, int R, int W, bool SPLIT>
-__global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R>()))
+template <typename T, int R, int W, int XM>
+__global__ void __launch_bounds__(64 << W, (sweep_waves_per_eu<T, R, XM>()))
 sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ blocks,
              const dq_phase *__restrict__ phases, const unsigned char *__restrict__ prog,
              const uint32_t *__restrict__ phase_aux, const uint32_t *__restrict__ block_aux) {
@@ -321,7 +321,7 @@ sweep_kernel(const dq_job *__restrict__ jobs, const dq_block *__restrict__ block
     const uint64_t base = tile_base_runs(block_aux + job.block * DQ_BAUX, blockIdx.x);
     Regs<T, R> a;
     load_tile<T, R, W>(a, blk, st, base, wv, lane);
-    process_tile<T, R, W, SPLIT>(a, phases + blk->phase_begin, blk->phase_count, phase_aux + blk->phase_begin * DQ_PAUX,
+    process_tile<T, R, W, XM>(a, phases + blk->phase_begin, blk->phase_count, phase_aux + blk->phase_begin * DQ_PAUX,
                                  prog, base | job.base_hi, tid, wv, false);
     if (store_needs_barrier<W>(blk)) __syncthreads();
     DQ_STORE_TILE(a, base);
@@ -830,8 +830,13 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
         rg = item / NCH;
         const uint32_t lz = dswz(coloff[ch * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u));
         const float4 *__restrict__ ma = reinterpret_cast<const float4 *>(mat) + (size_t)(rg * RB) * NQ * 64 + lane;
-        float4 cur[RB], nxt[RB];
-        float bc[4], bn[4];
+        // A operands (from L2) and B operands (LDS) of group q + 1 load while
+        // group q's MFMAs run, into the other slot of a two-slot ring: with
+        // the loop unrolled by 2 each slot is a fixed register set (no copies
+        // that would wait for the loads), and a scheduling barrier keeps the
+        // loads ahead of the MFMAs.
+        float4 ab[2][RB];
+        float bb[2][4];
         uint32_t x = 0;
         // B rows of group q: four steps of x over the real half (q < NQ/2),
         // then again from 0 over the imaginary half
@@ -845,24 +850,21 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
             }
         };
 #pragma unroll
-        for (int r = 0; r < RB; r++) cur[r] = ma[(size_t)r * NQ * 64];
-        read_b(0, bc);
+        for (int r = 0; r < RB; r++) ab[0][r] = ma[(size_t)r * NQ * 64];
+        read_b(0, bb[0]);
 #pragma unroll 2
         for (int q = 0; q < NQ; q++) {
-            if (q + 1 < NQ) {  // the next group's operands load while this one's MFMAs run
+            if (q + 1 < NQ) {
 #pragma unroll
-                for (int r = 0; r < RB; r++) nxt[r] = ma[((size_t)r * NQ + q + 1) * 64];
-                read_b(q + 1, bn);
+                for (int r = 0; r < RB; r++) ab[(q + 1) & 1][r] = ma[((size_t)r * NQ + q + 1) * 64];
+                read_b(q + 1, bb[(q + 1) & 1]);
             }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int r = 0; r < RB; r++)
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[r][i], bc[i], acc[r], 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < RB; r++) cur[r] = nxt[r];
-#pragma unroll
-            for (int i = 0; i < 4; i++) bc[i] = bn[i];
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(ab[q & 1][r][i], bb[q & 1][i], acc[r], 0, 0, 0);
         }
     }
     __syncthreads();  // every wave has read the whole tile: write in place
@@ -894,9 +896,10 @@ __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restri
     }
 }
 
-// One dense block over every tile of an FP32 state: grid = tiles, 512 threads.
+// One dense block over every tile of an FP32 state: grid = tiles, 512 threads;
+// at most 128 VGPRs so that two workgroups (64 KiB of LDS each) share a CU.
 template <int KL>
-__global__ void __launch_bounds__(DENSE_THREADS)
+__global__ void __launch_bounds__(DENSE_THREADS, 4)
 dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restrict__ ops, const float *__restrict__ mats,
              const uint32_t *__restrict__ ctab, typename V2<float>::type *__restrict__ st) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -906,17 +909,27 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     float *im = re + NT;
     const uint32_t tid = threadIdx.x;
     const uint64_t base = tile_base(blk->nontile_mask, blockIdx.x);
+    // tile_q read as four words (scalar loads; byte loads would put these
+    // uniform values in vector registers)
+    const uint32_t *__restrict__ tq32 = reinterpret_cast<const uint32_t *>(blk->tile_q);
+    uint32_t tq[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) tq[k] = tq32[k];
     uint64_t hq[DENSE_T_MAX];
 #pragma unroll
-    for (int b = 0; b < DENSE_T_MAX; b++) hq[b] = 1ull << blk->tile_q[b];
+    for (int b = 0; b < DENSE_T_MAX; b++) hq[b] = 1ull << ((tq[b >> 2] >> (8 * (b & 3))) & 0xffu);
     // Element e = tid + 512 i of the tile holds the amplitude whose qubits are
     // e's bits placed on tile_q (bits 0..5 are qubits 0..5).  The placement and
     // the LDS swizzle are linear, so the thread's part (bits 0..8) is formed
     // once and the part of i (bits 9 and up) is a scalar.
-    uint64_t tg = base | (tid & 63u);
+    // Bits 6..8 of e are the wave index, a scalar: the thread's part of an
+    // address is its lane alone (qubits 0..5), a 32-bit offset on a scalar base.
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t lane = tid & 63u;
+    uint64_t tg = base;
 #pragma unroll
     for (int b = 6; b < 9; b++)
-        if ((tid >> b) & 1u) tg |= hq[b];
+        if ((wv >> (b - 6)) & 1u) tg |= hq[b];
     const uint32_t tz = dswz(tid);
     // i < NI = 2^(T - 9) (2..16) runs over the tile bits 9 and up, whose qubits
     // are scalars.  Every thread issues 16 loads before using the first;
@@ -931,7 +944,7 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
     {
         typename V2<float>::type v[16];
 #pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = __builtin_nontemporal_load(st + gaddr((uint32_t)i & (NI - 1)));
+        for (int i = 0; i < 16; i++) v[i] = __builtin_nontemporal_load(lane_ptr(st + gaddr((uint32_t)i & (NI - 1)), lane));
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t z = tz ^ dswz(((uint32_t)i & (NI - 1)) * DENSE_THREADS);
@@ -953,8 +966,12 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
             v[i].x = re[z];
             v[i].y = im[z];
         }
+        // an opaque copy of the lane: the store addresses are formed again here
+        // instead of the load addresses being kept live through every op
+        uint32_t lane2 = lane;
+        asm volatile("" : "+v"(lane2));
 #pragma unroll
-        for (int i = 0; i < 16; i++) __builtin_nontemporal_store(v[i], st + gaddr((uint32_t)i & (NI - 1)));
+        for (int i = 0; i < 16; i++) __builtin_nontemporal_store(v[i], lane_ptr(st + gaddr((uint32_t)i & (NI - 1)), lane2));
     }
 }
 
@@ -1030,8 +1047,8 @@ struct SweepTable {
     SweepTable() {
         memset(fn, 0, sizeof fn);
 #define DQ_INST(R_, W_)                                                                  \
-    fn[0][R_][W_] = sweep_kernel<T, R_, W_, false>;                                      \
-    fn[1][R_][W_] = sweep_kernel<T, R_, W_, true>;
+    fn[0][R_][W_] = sweep_kernel<T, R_, W_, 0>;                                          \
+    fn[1][R_][W_] = sweep_kernel<T, R_, W_, 1>;
         DQ_INST(1, 0) DQ_INST(2, 0) DQ_INST(3, 0) DQ_INST(4, 0) DQ_INST(5, 0)
         DQ_INST(2, 1) DQ_INST(3, 1) DQ_INST(4, 1) DQ_INST(5, 1)
         DQ_INST(2, 2) DQ_INST(3, 2) DQ_INST(4, 2) DQ_INST(5, 2)
@@ -1046,27 +1063,35 @@ struct SweepTable {
 static SweepTable<double> g_tab64;
 static SweepTable<float> g_tab32;
 
-// split exchanges: on for every FP32 tile and for FP64 tiles of 12+ bits
-// unless DGQ_SPLIT_EXCHANGE=0 (=1 forces them on).  Half the LDS per
-// workgroup lets more FP32 workgroups share a CU (measured 18% faster on
-// 30-qubit FP32 circuits); FP64 tiles are bounded by registers instead.
+// Exchange mode of a tile shape (see exchange): split exchanges (1) for every
+// FP32 tile and for FP64 tiles of 12+ bits; half the LDS per workgroup lets
+// more FP32 workgroups share a CU (measured 18% faster on 30-qubit FP32
+// circuits).  DGQ_SPLIT_EXCHANGE: 0 whole amplitudes everywhere, 1 split
+// everywhere, unset the default.
 static int g_split_mode = -1;
-static bool use_split(int prec, int T) {
+static int exchange_mode(int prec, int R, int W, int T) {
+    (void)R;
+    (void)W;
     if (g_split_mode < 0) {
         const char *e = getenv("DGQ_SPLIT_EXCHANGE");
         g_split_mode = e ? atoi(e) : 2;
     }
-    if (g_split_mode == 0) return false;
-    if (g_split_mode == 1) return true;
-    return prec == DQ_F32 || T >= 12;
+    if (g_split_mode == 0) return 0;
+    if (g_split_mode == 1) return 1;
+    return (prec == DQ_F32 || T >= 12) ? 1 : 0;
 }
 
-static sweep_fn lookup(int prec, int R, int W, bool split = false) {
-    if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS) return nullptr;
-    return prec == DQ_F64 ? g_tab64.fn[split][R][W] : g_tab32.fn[split][R][W];
+static sweep_fn lookup(int prec, int R, int W, int xm) {
+    if (R < 0 || R > DQ_MAX_REG_BITS || W < 0 || W > DQ_MAX_WAVE_BITS || xm < 0 || xm > 1) return nullptr;
+    return prec == DQ_F64 ? g_tab64.fn[xm][R][W] : g_tab32.fn[xm][R][W];
 }
 
 static size_t elem_bytes(int prec) { return prec == DQ_F64 ? 16 : 8; }
+
+// bytes per LDS element of an exchange in mode xm
+static uint32_t exchange_elem_bytes(int prec, int xm) {
+    return xm == 0 ? (uint32_t)elem_bytes(prec) : (uint32_t)elem_bytes(prec) / 2;
+}
 
 // ----------------------------------------------------------------------------
 // staging of descriptors: a pool of pinned host buffers guarded by events.
@@ -1189,14 +1214,20 @@ extern "C" int dq_device_info(char *name, size_t name_len, int *num_cus, size_t 
 extern "C" int dq_malloc(void **ptr, size_t bytes) {
     if (!ptr) return fail(DQ_ERR_ARG, "dq_malloc: null out pointer");
     hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
-    if (e == hipErrorOutOfMemory) return fail(DQ_ERR_NOMEM, "dq_malloc: out of device memory (%zu bytes)", bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();  // a caller may go on (smaller request, other memory): do not leave the error pending
+        return fail(DQ_ERR_NOMEM, "dq_malloc: out of device memory (%zu bytes)", bytes);
+    }
     HIPC(e);
     return DQ_OK;
 }
 extern "C" int dq_free(void *ptr) { HIPC(hipFree(ptr)); return DQ_OK; }
 extern "C" int dq_host_alloc(void **ptr, size_t bytes) {
     hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 16, hipHostMallocDefault);
-    if (e == hipErrorOutOfMemory) return fail(DQ_ERR_NOMEM, "dq_host_alloc: out of pinned memory (%zu bytes)", bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();  // the paging executor falls back to pageable memory: clear the pending error
+        return fail(DQ_ERR_NOMEM, "dq_host_alloc: out of pinned memory (%zu bytes)", bytes);
+    }
     HIPC(e);
     return DQ_OK;
 }
@@ -1818,7 +1849,8 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             if (b.n_qubits != b0.n_qubits || b.reg_bits != b0.reg_bits || b.wave_bits != b0.wave_bits || b.tile_bits != b0.tile_bits)
                 return fail(DQ_ERR_ARG, "launch %u: jobs differ in shape", l);
         }
-        if (b0.reg_bits && !lookup(prec, b0.reg_bits, b0.wave_bits))
+        if (b0.reg_bits && !lookup(prec, b0.reg_bits, b0.wave_bits,
+                                   exchange_mode(prec, (int)b0.reg_bits, (int)b0.wave_bits, (int)b0.tile_bits)))
             return fail(DQ_ERR_UNSUPPORTED, "no sweep kernel for R=%d W=%d", b0.reg_bits, b0.wave_bits);
     }
 
@@ -1833,7 +1865,7 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
         if (b.reg_bits == 0) continue;
         if (!block_runs(b, baux.data() + (size_t)i * DQ_BAUX))
             return fail(DQ_ERR_UNSUPPORTED, "block %u: non-tile qubits in more than %d runs", i, DQ_RUNS);
-        const uint32_t es = (uint32_t)(use_split(prec, (int)b.tile_bits) ? elem_bytes(prec) / 2 : elem_bytes(prec));
+        const uint32_t es = exchange_elem_bytes(prec, exchange_mode(prec, (int)b.reg_bits, (int)b.wave_bits, (int)b.tile_bits));
         for (uint32_t p = b.phase_begin; p < b.phase_begin + b.phase_count; p++) {
             uint32_t *pa = paux.data() + (size_t)p * DQ_PAUX;
             if (prog_r[p] != (int)b.reg_bits || prog_es[p] != es) {
@@ -1914,13 +1946,14 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
         } else {
             bool multi = false;
             for (uint32_t j = j0; j < j0 + nj; j++) multi |= blocks[jobs[j].block].phase_count > 1;
-            const bool split = use_split(prec, b0.tile_bits);
-            const size_t lds = multi ? ((size_t)1 << b0.tile_bits) * (split ? elem_bytes(prec) / 2 : elem_bytes(prec)) : 0;
+            const int xm = exchange_mode(prec, (int)b0.reg_bits, (int)b0.wave_bits, (int)b0.tile_bits);
+            const size_t lds = multi ? ((size_t)1 << b0.tile_bits) * exchange_elem_bytes(prec, xm) : 0;
             if (lds > 131072) return fail(DQ_ERR_UNSUPPORTED, "tile of %d bits needs %zu bytes of LDS", b0.tile_bits, lds);
             const uint64_t tiles = 1ull << (n - b0.tile_bits);
             if (tiles > 0x7fffffffull || nj > 65535 || ((tiles << (6 + b0.wave_bits)) >> 32))
                 return fail(DQ_ERR_UNSUPPORTED, "grid too large (at most 2^32 threads per launch)");
-            sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, split);
+            sweep_fn fn = lookup(prec, b0.reg_bits, b0.wave_bits, xm);
+            if (!fn) return fail(DQ_ERR_UNSUPPORTED, "no sweep kernel for R=%u W=%u exchange mode %d", b0.reg_bits, b0.wave_bits, xm);
             hipLaunchKernelGGL(fn, dim3((unsigned)tiles, nj), dim3(64u << b0.wave_bits), lds, (hipStream_t)stream,
                                d_jobs + j0, d_blocks, d_phases, d_prog, d_paux, d_baux);
         }

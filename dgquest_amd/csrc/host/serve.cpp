@@ -956,6 +956,38 @@ int dgs_run_plain(const dgs_config *cfg, const dgs_request *reqs, uint32_t count
     if (rc) return fail(rc, "stream: %s", dq_last_error());
     void *host = nullptr;
     size_t host_cap = 0;
+    // Device memory for the largest request, allocated once before the clock
+    // starts and reused by every request: a one-at-a-time loop that keeps its
+    // register memory.  (Allocating and freeing per request, as createQureg /
+    // destroyQureg would, timed the HIP allocator too: single requests then
+    // stalled for up to seconds and the loop's rate varied 2x between runs.)
+    size_t dev_cap = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        Req r;
+        dgs_request q = reqs[i];
+        q.flags = DGS_NO_RECHECK;
+        if (make_req(q, cfg->prec, 0.0, r)) continue;  // reported in the timed loop
+        try {
+            plan_req(r, cfg->prec, *cfg);
+        } catch (const std::exception &) {
+            continue;
+        }
+        Item it;
+        it.reqs.push_back(&r);
+        it.n = r.n;
+        it.prec = cfg->prec;
+        it.shots = r.shots;
+        layout(it);
+        dev_cap = std::max(dev_cap, it.bytes);
+    }
+    void *dev = nullptr;
+    if (dev_cap) {
+        rc = dq_malloc(&dev, dev_cap);
+        if (rc) {
+            dq_stream_destroy(stream);
+            return fail(rc, "dgs_run_plain: device memory: %s", dq_last_error());
+        }
+    }
     const auto start = Clock::now();
     size_t soff = 0, poff = 0;
     int first = DQ_OK;
@@ -978,7 +1010,6 @@ int dgs_run_plain(const dgs_config *cfg, const dgs_request *reqs, uint32_t count
             }
         }
         Item it;
-        void *dev = nullptr;
         if (!rc) {
             it.reqs.push_back(&r);
             it.n = r.n;
@@ -993,8 +1024,7 @@ int dgs_run_plain(const dgs_config *cfg, const dgs_request *reqs, uint32_t count
                 if (!rc) host_cap = it.res_bytes;
             }
         }
-        // the register's own allocation, as createQureg / destroyQureg
-        if (!rc) rc = dq_malloc(&dev, it.bytes);
+        if (!rc && it.bytes > dev_cap) rc = fail(DQ_ERR_ARG, "dgs_run_plain: request %u larger than planned", i);
         if (!rc) rc = dq_event_create(&it.ev);
         if (!rc) {
             it.host = (unsigned char *)host;
@@ -1010,10 +1040,7 @@ int dgs_run_plain(const dgs_config *cfg, const dgs_request *reqs, uint32_t count
         if (it.ev) dq_event_destroy(it.ev);
         it.ev = nullptr;
         it.host = nullptr;
-        if (dev) {
-            dq_stream_sync(stream);
-            dq_free(dev);
-        }
+        dq_stream_sync(stream);
         R.id = i + 1;
         R.status = rc;
         R.n_qubits = reqs[i].n_qubits;
@@ -1027,6 +1054,7 @@ int dgs_run_plain(const dgs_config *cfg, const dgs_request *reqs, uint32_t count
         rc = DQ_OK;
     }
     if (host) dq_host_free(host);
+    if (dev) dq_free(dev);
     dq_stream_destroy(stream);
     return first;
 }

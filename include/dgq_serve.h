@@ -122,10 +122,12 @@ int dgs_replay(dgs_server *s, const dgs_request *reqs, uint32_t count, const dou
                uint64_t *samples, double *p0);
 
 /* The plain loop the server is measured against: one request at a time on
- * one stream, in arrival order, each with its own state allocation (as
- * createQureg / destroyQureg), plan, sweeps, read-out and a wait for the
- * result before the next starts.  Same arguments as dgs_replay; prec is the
- * state precision (the plain method is DQ_F64). */
+ * one stream, in arrival order, each planned, swept and read out with a wait
+ * for the result before the next starts.  Device memory for the largest
+ * request is allocated once before the loop's clock starts and reused (a
+ * loop that keeps its register memory, not a createQureg / destroyQureg per
+ * request).  Same arguments as dgs_replay; prec is the state precision (the
+ * plain method is DQ_F64). */
 int dgs_run_plain(const dgs_config *cfg, const dgs_request *reqs, uint32_t count, const double *arrive_s,
                   dgs_result *res, uint64_t *samples, double *p0);
 
