@@ -375,6 +375,40 @@ def serving(q64, args):
     }
 
 
+def api_default_step(q64, env, eng, circ, n, best_ms):
+    """The same circuit through the API with no shape given: the program
+    records it and calls searchQuregTaskPartitions, as the reference's
+    testcases do (testcases/bv-32.c:90), which plans every candidate shape
+    and keeps the least predicted device time.  Reports the shape picked, its
+    predicted and measured GPU ms per step, and the ratio to the bench's own
+    best (autotuned, or the --shape given)."""
+    import ctypes as C
+
+    q = q64.createQureg(n, env)
+    stream = q64.dgq_stream(q)
+    e0, e1 = eng.event(), eng.event()
+    q64.apply_circuit(q, circ)
+    q64.searchQuregTaskPartitions(q, env)
+    L = q64.lib
+    L.dgq_last_search.argtypes = [type(q), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    shape, pred = (C.c_int * 4)(), C.c_double()
+    searched = L.dgq_last_search(q, shape, C.byref(pred)) == 0
+    q64.dgq_flush(q)  # the first flush runs the plan the search kept
+    ms = []
+    for _ in range(2):
+        q64.initZeroState(q)
+        q64.apply_circuit(q, circ)
+        eng.record(e0, stream)
+        q64.dgq_flush(q)
+        eng.record(e1, stream)
+        ms.append(eng.elapsed_ms(e0, e1))
+    q64.destroyQureg(q, env)
+    m = min(ms)
+    return {"searched": searched, "shape": f"k{shape[0]}_R{shape[1]}_W{shape[2]}" + ("" if shape[3] else "_fixed_lanes"),
+            "predicted_ms": round(pred.value, 2), "gpu_ms_per_step": round(m, 2),
+            "vs_bench_best": round(m / best_ms, 4)}
+
+
 def serve_replay_only(q64, args):
     """The served replay of the serving mix alone (no plain loops, no recheck
     pass): what a kernel trace of the server's launch groups needs."""
@@ -548,6 +582,7 @@ def main():
     value = total_bytes / wall / 1e9
     achieved = total_bytes / (gpu_ms * 1e-3) / 1e9
     q64.destroyQureg(q, env)
+    api = api_default_step(q64, env, eng, circ, n, gpu_ms / args.steps)
 
     out = {
         "metric": METRIC,
@@ -577,6 +612,7 @@ def main():
             "lds_exchanges_per_step": st["exchanges"],
             "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
             "autotune_ms_per_shape": tuning,
+            "api_default": api,
         },
         "roofline": {
             "bound": "hbm",

@@ -179,3 +179,46 @@ def test_reference_testcases_compile_against_the_drop_in(tmp_path):
                str(out), "-L", LIB, "-lQuEST_f64", "-lm", f"-Wl,-rpath,{LIB}"]
         p = subprocess.run(cmd, capture_output=True, text=True)
         assert p.returncode == 0, p.stderr[-2000:]
+
+
+def _searched(n, circ):
+    """(shape, predicted ms, sweeps of the searched plan) for a recording register."""
+    from dgquest_amd import quest as Q
+    from dgquest_amd.quest import Qureg
+
+    q64 = Q.load(2)
+    L = q64.lib
+    L.dgq_last_search.argtypes = [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    q = L.dgq_create_recorder(n)
+    q64.apply_circuit(q, circ)
+    L.searchQuregTaskPartitions(q, Q.QuESTEnv())  # the search needs no device
+    sh, pm = (C.c_int * 4)(), C.c_double()
+    assert L.dgq_last_search(q, sh, C.byref(pm)) == 0  # the shape came from a search
+    return list(sh), pm.value
+
+
+def test_shape_search_on_the_benchmark_circuit():
+    """searchQuregTaskPartitions (src/cpp/QuEST_backend_distributed.cpp:86-121 picks a task
+    partition; here a fused-sweep shape): on the 30-qubit layered circuit the search keeps
+    the widest tile with rotating lanes -- the shape the bench's autotuner measures fastest
+    (24 sweeps against 30 with fixed lanes and 31 at k = 6) -- and every candidate's
+    predicted time is at least the chosen one's."""
+    from dgquest_amd import circuits
+
+    c = circuits.layered_u3_cz(30, 24, 2)
+    shape, pred = _searched(30, c)
+    assert shape == [7, 4, 3, 1]
+    assert pred > 0
+    # the prediction grows with the plan's sweeps: the same circuit at half the depth
+    half, pred_half = _searched(30, circuits.layered_u3_cz(30, 12, 2))
+    assert half == [7, 4, 3, 1] and 0.35 * pred < pred_half < 0.65 * pred
+
+
+def test_shape_search_fits_small_registers():
+    """A register narrower than a candidate's tile only considers the shapes that fit it."""
+    from dgquest_amd import circuits
+
+    for n in (10, 12, 14):
+        shape, pred = _searched(n, circuits.layered_u3_cz(n, 6, 1))
+        k, R, W, rot = shape
+        assert 6 + R + W <= n and k <= R + W and pred > 0
