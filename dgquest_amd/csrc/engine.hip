@@ -794,103 +794,122 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // only the low bits change, by a function of the others).
 __device__ __forceinline__ uint32_t dswz(uint32_t e) { return e ^ ((e >> 5) & 31u) ^ ((e >> 10) & 31u); }
 
-// One dense op on the tile in LDS.  Work item = (32 columns, RB row blocks of
-// 32); one item per wave at T = 13.  coloff[c] is column c's tile offset (c's
-// bits placed on the tile bits outside the op), a table the host forms per op.  The B rows of the op's bits 1..k-1 step
-// with the submask increment x' = ((x | ~S) + 1) & S (op bits ascending); the
-// LDS swizzle is linear, so a B address is the lane's column part xor the
-// swizzle of x, a scalar.  The A operands (16 bytes a lane per row block)
-// and the B operands of the next four MFMAs are loaded while the current four
-// run.
-template <int KL>
+// One dense op on the tile in LDS.  Work item = CB chunks of 32 columns x RB
+// row blocks of 32 rows (RB x CB = 2 result tiles); one item per wave at
+// T = 13.  k = 5 takes RB = 2, CB = 1: an item holds all 2^(k+1) rows of its
+// columns.  Wider ops take RB = 1, CB = 2 where the tile has two chunks: each
+// A operand (the matrix, from L2) then feeds two MFMAs, which halves the
+// matrix traffic per MFMA (for k = 7 every workgroup reads 256 KiB of matrix
+// per op).  coloff[c] is column c's tile offset (c's bits placed on the tile
+// bits outside the op), a table the host forms per op.  The B rows of the
+// op's bits 1..k-1 step with the submask increment x' = ((x | ~S) + 1) & S
+// (op bits ascending); the LDS swizzle is linear, so a B address is the
+// lane's column part xor the swizzle of x, a scalar.
+template <int KL, int CB>
 __device__ __forceinline__ void dense_op(float *__restrict__ re, float *__restrict__ im, uint32_t T,
                                          const dq_dense_op &op, const float *__restrict__ mat,
                                          const uint32_t *__restrict__ coloff, uint32_t tid) {
     constexpr int K = 1 << KL, M = 2 * K, MB = M / 32;
-    constexpr int RB = MB < 2 ? MB : 2;          // row blocks per work item
-    constexpr int NQ = K / 4;                    // groups of four contraction pairs (both halves)
+    constexpr int RB = CB == 2 ? 1 : (MB < 2 ? MB : 2);  // row blocks per work item
+    constexpr int NA = RB * CB;                           // result tiles per work item
+    constexpr int NQ = K / 4;                             // groups of four contraction pairs (both halves)
     const uint32_t NC = 1u << (T - KL);
-    const uint32_t NCH = NC / 32;
-    const uint32_t items = NCH * (MB / RB);
+    const uint32_t NCG = NC / (32 * CB);                  // column groups
+    const uint32_t items = NCG * (MB / RB);
     const uint32_t w = tid >> 6, lane = tid & 63;
     uint32_t used = 0;
     for (int j = 0; j < KL; j++) used |= 1u << op.bits[j];
     const uint32_t b0 = 1u << op.bits[0];
     const uint32_t S = used & ~b0;  // tile bits of matrix-index bits 1..k-1
     __syncthreads();  // the previous op's writes (or the tile load) are done
-    v16f acc[RB];
+    v16f acc[NA];
 #pragma unroll
-    for (int r = 0; r < RB; r++)
-        for (int v = 0; v < 16; v++) acc[r][v] = 0.0f;
+    for (int a = 0; a < NA; a++)
+        for (int v = 0; v < 16; v++) acc[a][v] = 0.0f;
     const uint32_t item = w;
     const bool busy = item < items;  // uniform over the wave
-    uint32_t rg = 0;
+    const uint32_t cg = item % NCG, rg = item / NCG;
     if (busy) {
-        const uint32_t ch = item % NCH;
-        rg = item / NCH;
-        const uint32_t lz = dswz(coloff[ch * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u));
+        uint32_t lz[CB];
+#pragma unroll
+        for (int c = 0; c < CB; c++)
+            lz[c] = dswz(coloff[(cg * CB + c) * 32 + (lane & 31u)] | ((lane >> 5) ? b0 : 0u));
         const float4 *__restrict__ ma = reinterpret_cast<const float4 *>(mat) + (size_t)(rg * RB) * NQ * 64 + lane;
-        // A operands (from L2) and B operands (LDS) of group q + 1 load while
-        // group q's MFMAs run, into the other slot of a two-slot ring: with
-        // the loop unrolled by 2 each slot is a fixed register set (no copies
-        // that would wait for the loads), and a scheduling barrier keeps the
-        // loads ahead of the MFMAs.
-        float4 ab[2][RB];
-        float bb[2][4];
+        // A operands load two groups ahead of their MFMAs into a four-slot
+        // ring, B operands (LDS) one group ahead into a two-slot ring: with the
+        // loop unrolled by 4 every slot is a fixed register set (no copies
+        // that would wait for the loads), and a scheduling barrier keeps each
+        // group's loads ahead of its MFMAs.
+        float4 ab[4][RB];
+        float bb[2][CB][4];
         uint32_t x = 0;
-        // B rows of group q: four steps of x over the real half (q < NQ/2),
-        // then again from 0 over the imaginary half
-        auto read_b = [&](int q, float *b) {
+        // B rows of group q for every chunk: four steps of x over the real
+        // half (q < NQ/2), then again from 0 over the imaginary half
+        auto read_b = [&](int q, float (*b)[4]) {
             if (q == NQ / 2) x = 0;
             const float *__restrict__ src = q < NQ / 2 ? re : im;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                b[i] = src[dswz(x) ^ lz];
+                const uint32_t sx = dswz(x);
+#pragma unroll
+                for (int c = 0; c < CB; c++) b[c][i] = src[sx ^ lz[c]];
                 x = ((x | ~S) + 1u) & S;
             }
         };
 #pragma unroll
-        for (int r = 0; r < RB; r++) ab[0][r] = ma[(size_t)r * NQ * 64];
+        for (int r = 0; r < RB; r++) {
+            ab[0][r] = ma[(size_t)r * NQ * 64];
+            ab[1][r] = ma[((size_t)r * NQ + 1) * 64];  // NQ >= 8
+        }
         read_b(0, bb[0]);
-#pragma unroll 2
+#pragma unroll 4
         for (int q = 0; q < NQ; q++) {
-            if (q + 1 < NQ) {
+            if (q + 2 < NQ) {
 #pragma unroll
-                for (int r = 0; r < RB; r++) ab[(q + 1) & 1][r] = ma[((size_t)r * NQ + q + 1) * 64];
-                read_b(q + 1, bb[(q + 1) & 1]);
+                for (int r = 0; r < RB; r++) ab[(q + 2) & 3][r] = ma[((size_t)r * NQ + q + 2) * 64];
             }
+            if (q + 1 < NQ) read_b(q + 1, bb[(q + 1) & 1]);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int r = 0; r < RB; r++)
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(ab[q & 1][r][i], bb[q & 1][i], acc[r], 0, 0, 0);
+#pragma unroll
+                    for (int c = 0; c < CB; c++)
+                        acc[r * CB + c] =
+                            __builtin_amdgcn_mfma_f32_32x32x2f32(ab[q & 3][r][i], bb[q & 1][c][i], acc[r * CB + c], 0, 0, 0);
         }
     }
-    __syncthreads();  // every wave has read the whole tile: write in place
+    // Write in place once every wave has read what it overwrites.  With
+    // RB = MB (k = 5) a work item holds all rows of its columns, so a wave
+    // reads exactly the tile elements it writes and program order suffices;
+    // otherwise a column's rows are split over several waves.
+    if constexpr (MB > RB) __syncthreads();
     if (busy) {
         // Result register v of this lane is row 32 rb + 8 (v / 4) + 4 (lane / 32)
         // + v % 4 (rb = rg RB + r); rows below K are real parts.  Its tile element
         // is the row's bits deposited on the op's bits, or the column's bits;
         // deposit and swizzle are linear over GF(2), so the LDS slot is the xor of
         // sd[b] = dswz(1 << bits[b]) over the row's set bits b, xor a lane part.
-        const uint32_t ch = item % NCH;
         uint32_t sd[KL];
 #pragma unroll
         for (int b = 0; b < KL; b++) sd[b] = dswz(1u << op.bits[b]);
-        const uint32_t lp = dswz(coloff[ch * 32 + (lane & 31u)]) ^ ((lane >> 5) ? sd[2] : 0u);
 #pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const uint32_t r0 = (rg * RB + r) * 32;  // the row block's first row (bits 5 and up)
-            uint32_t ur = 0;
+        for (int c = 0; c < CB; c++) {
+            const uint32_t lp = dswz(coloff[(cg * CB + c) * 32 + (lane & 31u)]) ^ ((lane >> 5) ? sd[2] : 0u);
 #pragma unroll
-            for (int b = 5; b < KL; b++) ur ^= ((r0 >> b) & 1u) ? sd[b] : 0u;
-            float *__restrict__ dst = (r0 >> KL) ? im : re;
+            for (int r = 0; r < RB; r++) {
+                const uint32_t r0 = (rg * RB + r) * 32;  // the row block's first row (bits 5 and up)
+                uint32_t ur = 0;
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const uint32_t u = ur ^ ((v & 1) ? sd[0] : 0u) ^ ((v & 2) ? sd[1] : 0u) ^ ((v & 4) ? sd[3] : 0u) ^
-                                   ((v & 8) ? sd[4] : 0u);
-                dst[u ^ lp] = acc[r][v];
+                for (int b = 5; b < KL; b++) ur ^= ((r0 >> b) & 1u) ? sd[b] : 0u;
+                float *__restrict__ dst = (r0 >> KL) ? im : re;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const uint32_t u = ur ^ ((v & 1) ? sd[0] : 0u) ^ ((v & 2) ? sd[1] : 0u) ^ ((v & 4) ? sd[3] : 0u) ^
+                                       ((v & 8) ? sd[4] : 0u);
+                    dst[u ^ lp] = acc[r * CB + c][v];
+                }
             }
         }
     }
@@ -953,9 +972,15 @@ dense_kernel(const dq_dense_block *__restrict__ blk, const dq_dense_op *__restri
         }
     }
     const size_t msz = (size_t)(2u << KL) * (2u << KL);
+    // ops wider than 5 bits pair two column chunks per work item when the
+    // tile has them (T >= k + 6; T is uniform over the launch)
+    const bool pair = KL >= 6 && T >= (uint32_t)KL + 6;
     for (uint32_t o = 0; o < blk->op_count; o++) {
         const dq_dense_op op = ops[blk->op_begin + o];
-        dense_op<KL>(re, im, T, op, mats + op.mat * msz, ctab + (size_t)(blk->op_begin + o) * DENSE_COLS, tid);
+        const float *m = mats + op.mat * msz;
+        const uint32_t *ct = ctab + (size_t)(blk->op_begin + o) * DENSE_COLS;
+        if (pair) dense_op<KL, 2>(re, im, T, op, m, ct, tid);
+        else dense_op<KL, 1>(re, im, T, op, m, ct, tid);
     }
     __syncthreads();
     {
