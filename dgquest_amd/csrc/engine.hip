@@ -73,15 +73,20 @@ struct Regs {
 //     this one, LDS byte offsets under that exchange's swizzle:
 //     [DQ_XA + j] / [DQ_XB + j] the uniform part of register index j in the
 //     previous / this phase's layout, [DQ_TA + i] / [DQ_TB + i] the part of
-//     thread bit i (lane bits 0..5, then wave bits).
+//     thread bit i (lane bits 0..5, then wave bits).  Split exchanges (mode
+//     1, owned slots) use the same words for their two computed sides, [2] = 1
+//     when the exchange needs its middle barrier, and [DQ_TF + i] /
+//     [DQ_FJ + j] for the thread's own slots.
 //   block_aux, DQ_BAUX words per block: the non-tile qubits as runs of
 //     consecutive qubits, [i] = first bit in the tile index | run length << 16,
 //     [16 + i] = the run's lowest qubit; runs past the last have length 0.
-#define DQ_PAUX 96
+#define DQ_PAUX 144
 #define DQ_XA 8
 #define DQ_XB 40
 #define DQ_TA 72
 #define DQ_TB 81
+#define DQ_TF 96
+#define DQ_FJ 105
 #define DQ_BAUX 32
 #define DQ_RUNS 10
 
@@ -113,23 +118,36 @@ __device__ __forceinline__ __attribute__((address_space(3))) T *lds_at(uint32_t 
 
 // Re-distribute the tile from the layout of the previous phase to that of
 // this one through LDS (each thread writes its 2^R amplitudes, then reads
-// 2^R).  Exchange modes (XM): 0 moves whole amplitudes; 1 moves the real
-// parts, then the imaginary parts, through an LDS array of scalars: half the
-// LDS footprint (more workgroups per CU), two more barriers.  An element's LDS slot is its tile index under a linear swizzle
-// the host picks per exchange (xor of the low bits with images of the high
-// bits) so that the lanes served together by the LDS in either layout hit
-// distinct banks; a byte offset is then the thread's part xor a uniform part
-// per register index j, both formed from the host's tables (ex = this phase's
-// phase_aux), one vector xor per access.
+// 2^R).  Exchange modes (XM):
+//  0  whole amplitudes.  An element's LDS slot is its tile index under a
+//     linear swizzle the host picks per exchange (xor of the low bits with
+//     images of the high bits) so that the lanes served together by the LDS in
+//     either layout hit distinct banks; a byte offset is the thread's part xor
+//     a uniform part per register index j, from the host's tables (ex = this
+//     phase's phase_aux), one vector xor per access.  A barrier leads every
+//     exchange after a block's first (other threads may still read the slots).
+//  1  real parts, then imaginary parts, through an LDS array of scalars (half
+//     the footprint: more workgroups per CU), in owned slots: every position
+//     (thread, register index) of a layout has a fixed slot, own[j] for this
+//     thread's register j, under one swizzle per block.  Real parts go from
+//     the old positions' slots to their new owners (ex's TB / XB parts: the
+//     slot of the previous layout's position of each new element), imaginary
+//     parts from their old owners to the new positions' slots (TA / XA).  The
+//     imaginary read is from the thread's own slots, which the next
+//     exchange's real write reuses, so no barrier leads an exchange; the one
+//     between the passes (ex[2]) is needed only when the layout change is not
+//     a set of pairwise swaps of positions, since only then may a thread
+//     write an imaginary part into a slot another thread has yet to read a
+//     real part from.
 template <int R, int W, int XM, typename T>
 __device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restrict__ ex, bool lead_barrier,
-                                         uint32_t lane, uint32_t wv) {
+                                         uint32_t lane, uint32_t wv, const uint32_t *own) {
     constexpr int NR = 1 << R;
     const uint32_t *__restrict__ xa = ex + DQ_XA;
     const uint32_t *__restrict__ xb = ex + DQ_XB;
     const uint32_t sa = thread_part<W>(ex + DQ_TA, lane, wv);
     const uint32_t sb = thread_part<W>(ex + DQ_TB, lane, wv);
-    if (XM == 0) {
+    if constexpr (XM == 0) {
         if (lead_barrier) __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) {
@@ -145,25 +163,18 @@ __device__ __forceinline__ void exchange(Regs<T, R> &a, const uint32_t *__restri
             a.im[j] = q[1];
         }
     } else {
-        if (lead_barrier) __syncthreads();
+        (void)lead_barrier;
 #pragma unroll
-        for (int j = 0; j < NR; j++) *lds_at<T>(sa ^ xa[j]) = a.re[j];
+        for (int j = 0; j < NR; j++) *lds_at<T>(own[j]) = a.re[j];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NR; j++) a.re[j] = *lds_at<T>(sb ^ xb[j]);
-        // for R <= 4 the imaginary pass forms its 2^(R+1) addresses again (one
-        // xor each) instead of keeping them in registers across the passes; at
-        // R = 5 recomputing them spills more than keeping them.  FP64 tiles of
-        // 8 waves keep them: two workgroups per CU (the LDS bound) already
-        // limit those to 4 waves per SIMD, so the extra registers are free
-        uint32_t sa2 = sa, sb2 = sb;
-        if constexpr (R <= 4 && !(std::is_same<T, double>::value && W == 3)) asm volatile("" : "+v"(sa2), "+v"(sb2));
+        if (ex[2]) __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NR; j++) *lds_at<T>(sa ^ xa[j]) = a.im[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NR; j++) *lds_at<T>(sa2 ^ xa[j]) = a.im[j];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < NR; j++) a.im[j] = *lds_at<T>(sb2 ^ xb[j]);
+        for (int j = 0; j < NR; j++) a.im[j] = *lds_at<T>(own[j]);
     }
 }
 
@@ -300,8 +311,19 @@ template <typename T, int R, int W, int XM>
 __device__ __forceinline__ void process_tile(Regs<T, R> &a, const dq_phase *__restrict__ ph, uint32_t nph,
                                              const uint32_t *__restrict__ pa, const unsigned char *__restrict__ prog,
                                              uint64_t cbase, uint32_t tid, uint32_t wv, bool lead) {
+    // owned-slot exchanges: the thread's own slots are the same in every
+    // exchange of a block, formed once
+    uint32_t own[XM == 1 ? (1 << R) : 1];
+    if constexpr (XM == 1) {
+        if (nph > 1) {
+            const uint32_t *__restrict__ e1 = pa + DQ_PAUX;
+            const uint32_t f = thread_part<W>(e1 + DQ_TF, tid & 63u, wv);
+#pragma unroll
+            for (int j = 0; j < (1 << R); j++) own[j] = f ^ e1[DQ_FJ + j];
+        }
+    }
     for (uint32_t p = 0; p < nph; p++) {
-        if (p > 0) exchange<R, W, XM, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid & 63u, wv);
+        if (p > 0) exchange<R, W, XM, T>(a, pa + p * DQ_PAUX, p > 1 || lead, tid & 63u, wv, own);
         AsmOps<T, R>::run(a, prog + pa[p * DQ_PAUX], cbase, tid);
     }
 }
@@ -1374,6 +1396,131 @@ static bool pick_swizzle(const dq_phase &A, const dq_phase &B, int L, uint32_t *
     return false;
 }
 
+// Owned-slot exchanges (split mode 1).  A position of a layout is (thread,
+// register index) as the bit vector lane bits 0..5, wave bits, register bits;
+// every position has a fixed LDS slot, the position under a swizzle that xors
+// images g[b] < 2^L of its high bits b >= L into its low L bits.  The bit of a
+// tile bit's position in a phase's layout:
+static int pos_bit(const dq_phase &P, uint32_t x, int R, int W) {
+    for (int l = 0; l < 6; l++)
+        if (P.lane_t[l] == x) return l;
+    for (int w = 0; w < W; w++)
+        if (P.wave_t[w] == x) return 6 + w;
+    for (int r = 0; r < R; r++)
+        if (P.reg_t[r] == x) return 6 + W + r;
+    return -1;
+}
+
+// The tile bit at position bit b of a layout.
+static uint32_t tile_bit_at(const dq_phase &P, int b, int W) {
+    if (b < 6) return P.lane_t[b];
+    if (b < 6 + W) return P.wave_t[b - 6];
+    return P.reg_t[b - 6 - W];
+}
+
+// One swizzle for all exchanges of a block (consecutive exchanges share the
+// slots of the layout between them).  A thread's own slots differ only in its
+// lane bits among the 2^L lanes the LDS serves together, so they never
+// conflict; the computed sides are the real read (the previous layout's
+// positions of the lanes' new elements) and the imaginary write (the next
+// layout's positions of their old ones).  Picks g so that on both sides of
+// every exchange the images of the lanes' low L bits are independent (search
+// as in pick_swizzle); false leaves g = 0 (correct, with bank conflicts).
+static bool pick_owned_swizzle(const dq_phase *ph, uint32_t nph, int R, int W, int L, uint32_t *g) {
+    memset(g, 0, 32 * sizeof(uint32_t));
+    std::vector<std::vector<int>> groups;
+    for (uint32_t p = 1; p < nph; p++) {
+        const dq_phase &A = ph[p - 1], &B = ph[p];
+        std::vector<int> g1, g2;
+        for (int l = 0; l < L; l++) {
+            g1.push_back(pos_bit(A, B.lane_t[l], R, W));
+            g2.push_back(pos_bit(B, A.lane_t[l], R, W));
+        }
+        groups.push_back(g1);
+        groups.push_back(g2);
+    }
+    std::vector<int> vars;
+    for (auto &gr : groups)
+        for (int b : gr)
+            if (b >= L && std::find(vars.begin(), vars.end(), b) == vars.end()) vars.push_back(b);
+    std::vector<bool> set(32, false);
+    auto ok = [&](const std::vector<int> &bits) {
+        uint32_t v[6];
+        int n = 0;
+        for (int b : bits) {
+            if (b < 0) continue;
+            if (b < L) v[n++] = 1u << b;
+            else if (set[b]) v[n++] = g[b];
+        }
+        return gf2_rank(v, n) == n;
+    };
+    std::vector<uint32_t> cand;
+    for (int i = 0; i < L; i++) cand.push_back(1u << i);
+    for (uint32_t c = 1; c < (1u << L); c++)
+        if (c & (c - 1)) cand.push_back(c);
+    long budget = 200000;
+    std::function<bool(size_t)> dfs = [&](size_t i) -> bool {
+        if (i == vars.size()) return true;
+        for (uint32_t c : cand) {
+            if (--budget < 0) return false;
+            g[vars[i]] = c;
+            set[vars[i]] = true;
+            bool good = true;
+            for (auto &gr : groups) good = good && ok(gr);
+            if (good && dfs(i + 1)) return true;
+            set[vars[i]] = false;
+        }
+        g[vars[i]] = 0;
+        return false;
+    };
+    if (dfs(0)) return true;
+    memset(g, 0, 32 * sizeof(uint32_t));
+    return false;
+}
+
+// Tables of the owned-slot exchange from layout A to layout B (phase_aux of
+// B): [DQ_TF] / [DQ_FJ] the thread's own slots, [DQ_TB] / [DQ_XB] the real
+// read (slot of the A-position of B's element), [DQ_TA] / [DQ_XA] the
+// imaginary write (slot of the B-position of A's element), [2] = 1 unless the
+// layout change is a set of pairwise swaps of position bits.
+static bool owned_tables(const dq_phase &A, const dq_phase &B, int R, int W, int L, const uint32_t *g, uint32_t es,
+                         uint32_t *pa) {
+    const int T = 6 + W + R;
+    auto sig = [&](int b) { return ((1u << b) ^ (b >= L ? g[b] : 0u)) * es; };
+    int to_b[32], to_a[32];  // position bit of A -> of B, and back
+    for (int b = 0; b < T; b++) {
+        to_b[b] = pos_bit(B, tile_bit_at(A, b, W), R, W);
+        to_a[b] = pos_bit(A, tile_bit_at(B, b, W), R, W);
+        if (to_b[b] < 0 || to_a[b] < 0) return false;
+    }
+    bool swaps = true;
+    for (int b = 0; b < T; b++) swaps &= to_b[to_b[b]] == b;
+    pa[2] = swaps ? 0u : 1u;
+    for (int l = 0; l < 6; l++) {
+        pa[DQ_TF + l] = sig(l);
+        pa[DQ_TB + l] = sig(to_a[l]);
+        pa[DQ_TA + l] = sig(to_b[l]);
+    }
+    for (int w = 0; w < W; w++) {
+        pa[DQ_TF + 6 + w] = sig(6 + w);
+        pa[DQ_TB + 6 + w] = sig(to_a[6 + w]);
+        pa[DQ_TA + 6 + w] = sig(to_b[6 + w]);
+    }
+    for (uint32_t j = 0; j < (1u << R); j++) {
+        uint32_t f = 0, xb = 0, xa = 0;
+        for (int r = 0; r < R; r++)
+            if ((j >> r) & 1) {
+                f ^= sig(6 + W + r);
+                xb ^= sig(to_a[6 + W + r]);
+                xa ^= sig(to_b[6 + W + r]);
+            }
+        pa[DQ_FJ + j] = f;
+        pa[DQ_XB + j] = xb;
+        pa[DQ_XA + j] = xa;
+    }
+    return true;
+}
+
 // Swizzled tile index of x under images g (bits b >= L).
 static uint32_t swizzle_host(uint32_t x, const uint32_t *g, int L) {
     uint32_t y = x;
@@ -1890,7 +2037,15 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
         if (b.reg_bits == 0) continue;
         if (!block_runs(b, baux.data() + (size_t)i * DQ_BAUX))
             return fail(DQ_ERR_UNSUPPORTED, "block %u: non-tile qubits in more than %d runs", i, DQ_RUNS);
-        const uint32_t es = exchange_elem_bytes(prec, exchange_mode(prec, (int)b.reg_bits, (int)b.wave_bits, (int)b.tile_bits));
+        const int bxm = exchange_mode(prec, (int)b.reg_bits, (int)b.wave_bits, (int)b.tile_bits);
+        const uint32_t es = exchange_elem_bytes(prec, bxm);
+        uint32_t gown[32];
+        if (bxm == 1) {
+            if (b.tile_bits != 6 + b.wave_bits + b.reg_bits)
+                return fail(DQ_ERR_ARG, "block %u: tile of %u bits is not 6 + %u + %u", i, b.tile_bits, b.wave_bits, b.reg_bits);
+            const int Lo = es == 4 ? 5 : (es == 8 ? 4 : 3);
+            pick_owned_swizzle(phases + b.phase_begin, b.phase_count, (int)b.reg_bits, (int)b.wave_bits, Lo, gown);
+        }
         for (uint32_t p = b.phase_begin; p < b.phase_begin + b.phase_count; p++) {
             uint32_t *pa = paux.data() + (size_t)p * DQ_PAUX;
             if (prog_r[p] != (int)b.reg_bits || prog_es[p] != es) {
@@ -1907,6 +2062,19 @@ extern "C" int dq_apply_blocks(int prec, const dq_block *blocks, uint32_t n_bloc
             // the exchange from phase p - 1: its swizzle and offset tables
             const dq_phase &A = phases[p - 1], &B = phases[p];
             const int L = es == 4 ? 5 : (es == 8 ? 4 : 3);
+            if (bxm == 1) {
+                uint32_t t[DQ_PAUX];
+                memcpy(t, pa, sizeof t);
+                if (!owned_tables(A, B, (int)b.reg_bits, (int)b.wave_bits, L, gown, es, t))
+                    return fail(DQ_ERR_ARG, "phase %u: layouts of the exchange do not place the same tile bits", p);
+                // a phase shared with an earlier block must get the same tables
+                bool filled = false;
+                for (int k = 2; k < DQ_PAUX; k++) filled |= pa[k] != 0;
+                if (filled && memcmp(t + 2, pa + 2, (DQ_PAUX - 2) * sizeof(uint32_t)))
+                    return fail(DQ_ERR_ARG, "phase %u: shared by blocks with different exchange slots", p);
+                memcpy(pa, t, sizeof t);
+                continue;
+            }
             uint32_t g[32];
             pick_swizzle(A, B, L, g);
             for (uint32_t j = 0; j < (1u << b.reg_bits); j++) {

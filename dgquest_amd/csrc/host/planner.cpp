@@ -568,6 +568,32 @@ Layout layout_for(uint32_t reg_tbits, int T, int R, int W) {
     return L;
 }
 
+// Layout of the next phase as pairwise swaps of A: register bits that stay
+// targets keep their slots; each new target takes the slot of a register bit
+// that is no longer one, which moves to the target's old lane or wave position.
+// An exchange between such layouts needs no barrier between its passes
+// (owned-slot exchanges).
+Layout swap_layout(const Layout &A, uint32_t reg_tbits, int T, int R, int W) {
+    Layout L = A;
+    uint32_t in_regs = 0;
+    for (int r = 0; r < R; r++) in_regs |= 1u << A.reg[r];
+    int free_slots[DQ_MAX_REG_BITS], nf = 0;
+    for (int r = 0; r < R; r++)
+        if (!((reg_tbits >> A.reg[r]) & 1)) free_slots[nf++] = r;
+    int k = 0;
+    for (int b = 0; b < T && k < nf; b++) {
+        if (!((reg_tbits >> b) & 1) || ((in_regs >> b) & 1)) continue;
+        const int r = free_slots[k++];
+        const uint8_t y = A.reg[r];
+        for (int l = 0; l < 6; l++)
+            if (L.lane[l] == b) L.lane[l] = y;
+        for (int w = 0; w < W; w++)
+            if (L.wave[w] == b) L.wave[w] = y;
+        L.reg[r] = (uint8_t)b;
+    }
+    return L;
+}
+
 dq_phase phase_of(const Layout &L, int R, int W) {
     dq_phase ph;
     memset(&ph, 0, sizeof ph);
@@ -673,7 +699,8 @@ void emit_block(const std::vector<Gate> &bg, uint64_t tmask, int n, int nl, cons
     std::vector<const std::vector<Gate> *> opsets;
     static const std::vector<Gate> kNone;
     for (size_t i = 0; i < sets.size(); i++) {
-        Layout L = layout_for(set_tbits[i], T, R, W);
+        // after the first phase, pairwise swaps of the phase before
+        Layout L = i > 0 ? swap_layout(lays.back(), set_tbits[i], T, R, W) : layout_for(set_tbits[i], T, R, W);
         if (i == 0 && !lanes_identity(L)) { lays.push_back(standard_layout(T, R, W)); opsets.push_back(&kNone); }
         lays.push_back(L);
         opsets.push_back(&sets[i]);
