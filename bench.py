@@ -32,6 +32,8 @@ are measured work-bound), their ratio, p50/p99 latency, and latency at half
 the served capacity.  paged_bv: a Bernstein-Vazirani register forced into pinned
 host pieces by a small device budget.  paged_layered: a layered circuit paged
 the same way, whose sweeps give the copies something to overlap with.
+resident_bv: Bernstein-Vazirani on a 34-qubit FP64 register held in HBM
+(256 GiB), the largest that fits one MI355X.
 """
 import argparse
 import json
@@ -75,6 +77,8 @@ def parse():
     ap.add_argument("--serve-dump", type=str, default="", help="write per-request results (JSON lines) here")
     ap.add_argument("--serve-replay-only", action="store_true",
                     help="run only the served replay of the serving mix (to trace its launch groups) and exit")
+    ap.add_argument("--resident-bv", type=int, default=34,
+                    help="qubits of the Bernstein-Vazirani run held resident in HBM (0 = skip)")
     ap.add_argument("--precision-qubits", type=int, default=32,
                     help="grid-circuit size of the FP32 MFMA precision-form run (0 = skip)")
     ap.add_argument("--precision-k", type=str, default="5,6,7")
@@ -261,6 +265,50 @@ def paged_bv(q64, env, n_data, budget_bytes, seed=3):
     }
     out.update(overlap_figures(st))
     return out
+
+
+def resident_bv(q64, env, eng, n, seed=3):
+    """Bernstein-Vazirani on the largest FP64 register one MI355X holds in HBM
+    (34 qubits: 256 GiB of its 288 GiB): the flush, then a measurement of every
+    data qubit (deterministic outcomes: one P(0) pass serves them all)."""
+    import ctypes as C
+
+    from dgquest_amd import circuits
+
+    need = 16 << n
+    free = eng.info()["free_mem"]
+    # createQureg exits on an allocation failure (QuEST semantics) and puts a
+    # register above the device budget (free memory less 4 GiB) into pinned
+    # host pieces instead: check both before creating it
+    if free - (4 << 30) < need:
+        return {"skipped": f"{free >> 30} GiB free on the device, {need >> 30} GiB needed + 4 GiB margin"}
+    p = C.c_void_p()
+    if eng.lib.dq_malloc(C.byref(p), need) != 0:
+        return {"skipped": f"a trial allocation of {need >> 30} GiB failed"}
+    eng.lib.dq_free(p)
+    circ, hidden = circuits.bernstein_vazirani(n - 1, seed)
+    t0 = time.perf_counter()
+    q = q64.createQureg(n, env)
+    t1 = time.perf_counter()
+    q64.apply_circuit(q, circ)
+    q64.dgq_flush(q)
+    q64.dgq_sync(q)
+    t2 = time.perf_counter()
+    st = q64.plan_stats(q)
+    q64.seed([seed])
+    got = 0
+    for i in range(n - 1):
+        if q64.measure(q, i):
+            got |= 1 << i
+    t3 = time.perf_counter()
+    tp = q64.calcTotalProb(q)
+    q64.destroyQureg(q, env)
+    return {
+        "workload": f"{n}-qubit Bernstein-Vazirani (seed {seed}), FP64, {need >> 30} GiB resident in HBM",
+        "recovered_hidden_string": got == hidden, "total_prob": tp,
+        "create_s": round(t1 - t0, 3), "flush_s": round(t2 - t1, 3), "sweeps": st["sweeps"],
+        "measure_all_s": round(t3 - t2, 3),
+    }
 
 
 def serving(q64, args):
@@ -583,6 +631,9 @@ def main():
     achieved = total_bytes / (gpu_ms * 1e-3) / 1e9
     q64.destroyQureg(q, env)
     api = api_default_step(q64, env, eng, circ, n, gpu_ms / args.steps)
+    # before the other legs: the largest allocation, made while device memory
+    # is least fragmented
+    res_bv = resident_bv(q64, env, eng, args.resident_bv) if args.resident_bv else None
 
     out = {
         "metric": METRIC,
@@ -637,6 +688,8 @@ def main():
     # register per request, and device allocations became ~10x slower for
     # small registers after the precision leg's 64 + 3 x 32 GiB registers had
     # been allocated and freed in the same process
+    if res_bv is not None:
+        out["resident_bv"] = res_bv
     if args.serve:
         out["serving"] = serving(q64, args)
     if args.paged_bv:

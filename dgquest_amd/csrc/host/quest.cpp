@@ -371,6 +371,7 @@ void ensure_probs(QuregState *s) {
     s->p0.assign(s->n + 1, 0.0);
     ENG(dq_prob_zero_all(s->d_state, s->n, s->prec, s->p0.data(), s->d_ws, s->ws_cap, s->stream));
     s->probs_dirty = false;
+    s->det_base = true;
 }
 
 // Move every logical qubit back to its own physical position with
@@ -646,6 +647,7 @@ void setAmps(Qureg qureg, long long int startInd, qreal *reals, qreal *imags, lo
                           s->ws_cap, s->stream));
     }
     s->probs_dirty = true;
+    s->det_base = false;
 }
 
 // ---- gates ---------------------------------------------------------------
@@ -924,14 +926,16 @@ qreal collapseToOutcome(Qureg qureg, const int q, int outcome) {
     return prob;
 }
 
-// P(q = 0) of a paged register from its last P(0) pass when every gate
-// recorded since is the collapse of a deterministic measurement: each such
-// collapse drops a branch of probability at most its slack and rescales the
-// rest, which moves any P(q = 0) by at most 2 * slack in all.  Only a value
-// that decides the outcome rule the same way anywhere in that range is
-// taken (the paged P(0) pass reads every host piece over the host link).
-static bool paged_cached_zero_prob(QuregState *s, int q, double *zero_prob) {
-    if (!s->paged || !s->det_base || s->init_kind != DQ_INIT_LOAD) return false;
+// P(q = 0) from the register's last P(0) pass when every gate recorded since
+// is the collapse of a deterministic measurement: each such collapse drops a
+// branch of probability at most its slack and rescales the rest, which moves
+// any P(q = 0) by at most 2 * slack in all.  Only a value that decides the
+// outcome rule the same way anywhere in that range is taken.  A run of
+// deterministic measurements then costs one P(0) pass and one flush of the
+// collapses, instead of a flush and a pass over the whole state each (over
+// the host link for a paged register).
+static bool cached_zero_prob(QuregState *s, int q, double *zero_prob) {
+    if (!s->det_base || s->init_kind != DQ_INIT_LOAD) return false;
     if (s->pending.size() != (size_t)s->det_pending) return false;
     const double p = s->p0[s->l2p[q]], m = 2.0 * s->det_slack;
     if (!(p + m < REAL_EPS || 1.0 - (p - m) < REAL_EPS)) return false;
@@ -943,7 +947,7 @@ int measureWithStats(Qureg qureg, int q, qreal *outcomeProb) {
     v_target(qureg, q, __func__);
     QuregState *s = st(qureg);
     double cached = 0.0;
-    const qreal zeroProb = paged_cached_zero_prob(s, q, &cached) ? (qreal)cached : calcProbOfOutcome(qureg, q, 0);
+    const qreal zeroProb = cached_zero_prob(s, q, &cached) ? (qreal)cached : calcProbOfOutcome(qureg, q, 0);
     // src/QuEST_common.c:97-115
     int outcome;
     const bool determined = zeroProb < REAL_EPS || 1 - zeroProb < REAL_EPS;
@@ -952,8 +956,7 @@ int measureWithStats(Qureg qureg, int q, qreal *outcomeProb) {
     else outcome = (mt_genrand_real1() > zeroProb);
     const qreal prob = (outcome == 0) ? zeroProb : 1 - zeroProb;
     if (outcomeProb) *outcomeProb = prob;
-    const bool chain = s->paged && s->det_base && s->init_kind == DQ_INIT_LOAD &&
-                       s->pending.size() == (size_t)s->det_pending;
+    const bool chain = s->det_base && s->init_kind == DQ_INIT_LOAD && s->pending.size() == (size_t)s->det_pending;
     record_collapse(qureg, q, outcome, prob);
     if (chain && determined) {
         s->det_pending++;
@@ -1258,6 +1261,9 @@ void *dgq_state_pointer(Qureg q, int *prec) {
     if (!s || s->recorder || s->paged) return nullptr;
     materialize_mapping(s);
     ENG(dq_stream_sync(s->stream));
+    // the caller may write through the pointer: no cached P(0) survives it
+    s->probs_dirty = true;
+    s->det_base = false;
     if (prec) *prec = s->prec;
     return s->d_state;
 }

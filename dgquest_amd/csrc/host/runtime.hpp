@@ -62,7 +62,7 @@ struct QuregState {
         dgq::Plan plan;
     } cache;
     double predicted_ms = 0.0;        // searchQuregTaskPartitions' predicted device ms (0: no search)
-    // Paged registers: while det_base holds, p0 is exact for the state before
+    // While det_base holds, p0 is exact for the state before
     // the pending gates, and those are det_pending collapses of deterministic
     // measurements whose dropped branches held probability det_slack in all
     // (see measureWithStats).  Anything that changes the state clears det_base.
