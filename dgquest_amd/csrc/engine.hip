@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(256) prob_stage2(const double *__restrict__ pa
 #define NA_PART_DOUBLES ((1u << (NA_CHUNK_BITS - PROB_WG_BITS)) * PROB_SLOTS)
 #define NA_CHUNK_DOUBLES ((1u << (NA_MAX_QUBITS - NA_CHUNK_BITS)) * (NA_CHUNK_BITS + 1))
 static_assert((size_t)NA_PART_DOUBLES * sizeof(double) % 256 == 0, "chunk results stay 256-byte aligned");
-__device__ __align__(256) double scratch[NA_PART_DOUBLES + NA_CHUNK_DOUBLES];
+static __device__ __align__(256) double scratch[NA_PART_DOUBLES + NA_CHUNK_DOUBLES];
 
 // ----------------------------------------------------------------------------
 // read-out of many states of one size: P(0) per qubit, total probability and
