@@ -63,9 +63,10 @@ def parse():
     ap.add_argument("--shape", type=str, default="", help="k,R,W to skip the autotuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--paged-bv", type=int, default=28,
-                    help="qubits of the host-paged Bernstein-Vazirani run (0 = skip)")
-    ap.add_argument("--paged-budget-gib", type=float, default=1.0)
+    ap.add_argument("--paged-bv", type=int, default=31,
+                    help="qubits of the host-paged Bernstein-Vazirani run (0 = skip; 31 is the largest FP64 "
+                         "register whose pieces a host granting 64 GiB of pinned memory can hold)")
+    ap.add_argument("--paged-budget-gib", type=float, default=4.0)
     ap.add_argument("--paged-layered", type=str, default="30,6,4",
                     help="n,depth,budget_GiB of the host-paged layered U3+CZ run ('' = skip)")
     ap.add_argument("--serve", type=int, default=400, help="requests of the serving run (0 = skip)")
@@ -237,6 +238,17 @@ def paged_bv(q64, env, n_data, budget_bytes, seed=3):
     from dgquest_amd import circuits
 
     circ, hidden = circuits.bernstein_vazirani(n_data, seed)
+    # createQureg exits when it cannot pin every piece (QuEST semantics): pin
+    # the state's size once first and report a refusal instead
+    need = 16 << (n_data + 1)
+    L = q64.lib
+    L.dq_host_alloc.restype = C.c_int
+    L.dq_host_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.dq_host_free.argtypes = [C.c_void_p]
+    p = C.c_void_p()
+    if L.dq_host_alloc(C.byref(p), need) != 0:
+        return {"skipped": f"the host refused {need >> 30} GiB of pinned memory"}
+    L.dq_host_free(p)
     t_alloc = time.perf_counter()
     q = paged_register(q64, env, n_data + 1, budget_bytes)
     alloc_s = time.perf_counter() - t_alloc
