@@ -214,3 +214,28 @@ def test_paged_deterministic_measurements_then_random_one(quest, env, oracle):
     assert np.abs(quest.state_vector(q) - ref).max() <= TOL
     assert abs(quest.calcTotalProb(q) - 1.0) <= TOL
     quest.destroyQureg(q, env)
+
+
+@pytest.mark.parametrize("n,seed", [(15, 4), (16, 5)])
+def test_paged_fp32_register(quest32, env32, oracle, n, seed):
+    """FP32 amplitudes (8-byte elements) through the paging executor: the
+    oracle's FP64 state within FP32 rounding (2e-6 per amplitude over 150
+    gates), total probability within 1e-5, and the same circuit resident in
+    FP32 within the same bound."""
+    rng = np.random.default_rng(seed)
+    circ = H.random_circuit(rng, n, 150)
+    q, info = paged_register(quest32, env32, n, 32 << 10)
+    quest32.apply_circuit(q, circ)
+    sv_p = quest32.state_vector(q)
+    st = paged_info(quest32, q)
+    print(f"FP32 n={n} nl={info['nl']} K={info['K']} stats={st}", flush=True)
+    r = quest32.createQureg(n, env32)
+    quest32.apply_circuit(r, circ)
+    sv_r = quest32.state_vector(r)
+    ref = oracle_state(oracle, n, circ)
+    assert np.abs(sv_p - ref).max() <= 2e-6
+    assert np.abs(sv_p - sv_r).max() <= 2e-6
+    assert abs(quest32.calcTotalProb(q) - 1.0) <= 1e-5
+    assert st["remaps"] >= 1
+    quest32.destroyQureg(q, env32)
+    quest32.destroyQureg(r, env32)
